@@ -1,0 +1,171 @@
+"""RKS exchange-correlation engine on the GPU, mirroring ``calchem_dft::vxc``.
+
+RksVxcEngine(basis, grid)            RksVxcEngine::new_with_gpu (vxc.rs:429-716)
+engine.evaluate(basis, grid, xc, P)  RksVxcEngine::evaluate (vxc.rs:720-985) -> (E_xc, V)
+engine.evaluate_full(...)            same, plus N = sum_g w_g rho_g
+engine.telemetry()                   RksVxcEngine::telemetry (vxc.rs:420-425)
+GpuVxcGrid                           calchem_cuda::GpuVxcGrid (lib.rs:1178-1306): the
+                                     reference GPU backend contract (rho on the device,
+                                     functional on the host, V on the device).
+
+Differences from the reference, by design: there is no CPU fallback (a GPU
+failure raises CalchemHipError), the functional runs on the device in the
+fused build, and V is returned full-symmetric.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional, Tuple
+
+import numpy as np
+
+from . import _native
+from .basis import AO_COMP_DTYPE, AO_PRIM_DTYPE, AO_SHELL_DTYPE, BasisSet
+from .grid import MolecularGrid
+from .xc import xc_id
+
+PRECISIONS = {"fp64": 0, "fp32": 1}
+
+GRID_PT_DTYPE = np.dtype([("x", "<f8"), ("y", "<f8"), ("z", "<f8"), ("batch_id", "<i4"),
+                          ("host_atom", "<i4")], align=True)
+assert GRID_PT_DTYPE.itemsize == 32
+
+
+def _records(basis: BasisSet):
+    shells, prims, comps = basis.ao_records()
+    assert shells.dtype == AO_SHELL_DTYPE and prims.dtype == AO_PRIM_DTYPE and comps.dtype == AO_COMP_DTYPE
+    return np.ascontiguousarray(shells), np.ascontiguousarray(prims), np.ascontiguousarray(comps)
+
+
+class RksVxcEngine:
+    def __init__(self, basis: BasisSet, grid: MolecularGrid, block_points: int = 0,
+                 group_points: int = 0, memory_limit_bytes: int = 0):
+        self._lib = _native.require_device()
+        self.n_ao = basis.n_ao
+        self.n_points = grid.n_points
+        self.handle = 0
+        self._kernel_invocations = 0
+        shells, prims, comps = _records(basis)
+        xyzw = grid.xyzw()
+        opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes)
+        h = _native.new_handle()
+        rc = self._lib.calchem_hip_vxc_create(
+            _native.ptr(shells), len(shells), _native.ptr(prims), len(prims), _native.ptr(comps),
+            len(comps), _native.ptr(xyzw), xyzw.shape[0], self.n_ao, C.byref(opts), h)
+        _native.check(rc, "calchem_hip_vxc_create")
+        self.handle = h
+
+    @classmethod
+    def new_with_gpu(cls, basis: BasisSet, grid: MolecularGrid, use_gpu: bool = True) -> "RksVxcEngine":
+        if not use_gpu:
+            raise ValueError("calchem_amd has no CPU Vxc engine; use_gpu must be True")
+        return cls(basis, grid)
+
+    def close(self):
+        if self.handle:
+            self._lib.calchem_hip_vxc_release(self.handle)
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def evaluate_full(self, func: str, P: np.ndarray, precision: str = "fp64") -> Tuple[float, np.ndarray, float]:
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        n = self.n_ao
+        if P.shape != (n, n):
+            raise ValueError(f"density has shape {P.shape}, expected {(n, n)}")
+        V = np.empty((n, n))
+        e = C.c_double()
+        ne = C.c_double()
+        rc = self._lib.calchem_hip_vxc_build(self.handle, xc_id(func), PRECISIONS[precision],
+                                             _native.ptr(P), _native.ptr(V), C.byref(e), C.byref(ne))
+        _native.check(rc, "calchem_hip_vxc_build")
+        self._kernel_invocations += 2
+        return e.value, V, ne.value
+
+    def evaluate(self, basis: Optional[BasisSet], grid: Optional[MolecularGrid], func: str,
+                 density_total: np.ndarray, precision: str = "fp64") -> Tuple[float, np.ndarray]:
+        e, V, _ = self.evaluate_full(func, density_total, precision)
+        return e, V
+
+    def build_device(self, func: str, d_P: int, d_V: int, d_exc_nelec: int, stream: int = 0,
+                     precision: str = "fp64") -> None:
+        """Enqueue a build on device pointers (no synchronisation)."""
+        rc = self._lib.calchem_hip_vxc_build_device(self.handle, xc_id(func), PRECISIONS[precision],
+                                                    d_P, d_V, d_exc_nelec, stream or None)
+        _native.check(rc, "calchem_hip_vxc_build_device")
+        self._kernel_invocations += 2
+
+    def set_profiling(self, on: bool) -> None:
+        _native.check(self._lib.calchem_hip_vxc_set_profiling(self.handle, 1 if on else 0),
+                      "calchem_hip_vxc_set_profiling")
+
+    def stats(self) -> dict:
+        s = _native.VxcStats()
+        _native.check(self._lib.calchem_hip_vxc_stats(self.handle, C.byref(s)), "calchem_hip_vxc_stats")
+        return s.as_dict()
+
+    def telemetry(self) -> Tuple[int, int]:
+        """(GPU kernel invocations, CPU fallback invocations); the latter is always 0."""
+        return self._kernel_invocations, 0
+
+
+class GpuVxcGrid:
+    """The reference GPU backend contract: upload, eval_rho_grad, accumulate_vxc."""
+
+    def __init__(self, basis: BasisSet, grid: MolecularGrid, batch_size: int = 128,
+                 memory_limit_bytes: int = 0):
+        self._lib = _native.require_device()
+        self.n_ao = basis.n_ao
+        shells, prims, comps = _records(basis)
+        w = grid.weights
+        keep = np.nonzero(np.abs(w) >= 1e-24)[0]
+        self.keep = keep
+        n = len(keep)
+        pts = np.zeros(n, dtype=GRID_PT_DTYPE)
+        pts["x"], pts["y"], pts["z"] = (grid.coords[keep, 0], grid.coords[keep, 1], grid.coords[keep, 2])
+        pts["batch_id"] = np.arange(n) // batch_size
+        pts["host_atom"] = grid.points[keep, 5].astype(np.int32)
+        n_b = (n + batch_size - 1) // batch_size
+        # Every shell is a candidate of every batch: a valid (if loose) CSR.
+        cand_shells = np.tile(np.arange(len(shells), dtype=np.int32), n_b)
+        cand_offsets = (np.arange(n_b + 1, dtype=np.int32) * len(shells)).astype(np.int32)
+        self.n_pts = n
+        self.weights = np.ascontiguousarray(w[keep])
+        h = _native.new_handle()
+        rc = self._lib.calchem_hip_vxc_upload_grid(
+            _native.ptr(shells), len(shells), _native.ptr(prims), len(prims), _native.ptr(comps),
+            len(comps), _native.ptr(pts), n, _native.ptr(cand_offsets), len(cand_offsets),
+            _native.ptr(cand_shells), len(cand_shells), self.n_ao, h, memory_limit_bytes)
+        _native.check(rc, "calchem_hip_vxc_upload_grid")
+        self.grid_id = h
+
+    def eval_rho_grad(self, P: np.ndarray) -> np.ndarray:
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        out = np.zeros((self.n_pts, 4))
+        rc = self._lib.calchem_hip_vxc_eval_rho(self.grid_id, _native.ptr(P), _native.ptr(out), self.n_pts,
+                                                self.n_ao)
+        _native.check(rc, "calchem_hip_vxc_eval_rho")
+        return out
+
+    def accumulate_vxc(self, wv: np.ndarray) -> np.ndarray:
+        wv = np.ascontiguousarray(wv, dtype=np.float64)
+        V = np.zeros((self.n_ao, self.n_ao))
+        rc = self._lib.calchem_hip_vxc_accum(self.grid_id, _native.ptr(wv), _native.ptr(V), self.n_pts,
+                                             self.n_ao)
+        _native.check(rc, "calchem_hip_vxc_accum")
+        return V
+
+    def close(self):
+        if self.grid_id:
+            self._lib.calchem_hip_vxc_release(self.grid_id)
+            self.grid_id = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

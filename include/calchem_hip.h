@@ -1,0 +1,167 @@
+/*
+ * calchem_hip.h — C ABI of the MI355X (gfx950) DFT exchange-correlation
+ * quadrature engine: the per-SCF-iteration Vxc matrix, E_xc and integrated
+ * electron count N of CalChem's RKS path.
+ *
+ * Conventions (kept from the reference GPU backend): plain pointers and
+ * sizes, row-major FP64 matrices, caller-chosen 64-bit handles, int return
+ * codes 0 ok, -1 bad arguments, -2 shape or handle mismatch, -3 no usable
+ * device, -4 allocation failure, -5 launch or copy failure, -6 size limit
+ * exceeded, -7 unsupported option.  Reference paths are relative to the
+ * CalChem source tree.
+ */
+#ifndef CALCHEM_HIP_H
+#define CALCHEM_HIP_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Record layouts of crates/calchem-cuda/src/lib.rs:46-85 (AoShellGpu,
+ * AoPrimGpu, AoCompGpu, GridPtGpu), byte for byte. */
+typedef struct {
+    double center[3];      /* bohr */
+    double r2_cutoff;      /* (40 + 4 l) / alpha_min */
+    int32_t ao_offset;
+    int32_t n_funcs;
+    int32_t prim_start;
+    int32_t n_prims;
+    int32_t comp_start;
+    int32_t atom_index;
+} CalchemAoShell;
+
+typedef struct {
+    double exponent;
+    double weighted_norm;  /* normalised contraction coefficient x N(alpha; L,0,0) */
+} CalchemAoPrim;
+
+typedef struct {
+    double scale;          /* N(alpha_0; lx,ly,lz) / N(alpha_0; L,0,0) */
+    int32_t lx, ly, lz, _pad;
+} CalchemAoComp;
+
+typedef struct {
+    double x, y, z;
+    int32_t batch_id;
+    int32_t host_atom;
+} CalchemGridPt;
+
+/* Functional ids (crates/calchem-dft/src/xc.rs:16-35). */
+enum {
+    CALCHEM_XC_SVWN5 = 0,
+    CALCHEM_XC_BLYP = 1,
+    CALCHEM_XC_PBE = 2,
+    CALCHEM_XC_B3LYP = 3,
+    CALCHEM_XC_B3LYP5 = 4,
+    CALCHEM_XC_PBE0 = 5
+};
+
+/* Precision of the two contractions (AO values, phi*P and phi^T (w v) phi).
+ * The functional, per-point reductions and all sums across tiles are FP64
+ * in both modes. */
+enum { CALCHEM_PREC_FP64 = 0, CALCHEM_PREC_FP32 = 1 };
+
+/* Grid levels (crates/calchem-dft/src/grid.rs:13-20); EXPLICIT = fixed
+ * (n_rad, n_ang) per atom with no angular pruning. */
+enum {
+    CALCHEM_GRID_COARSE = 0,
+    CALCHEM_GRID_MEDIUM = 1,
+    CALCHEM_GRID_FINE = 2,
+    CALCHEM_GRID_ULTRAFINE = 3,
+    CALCHEM_GRID_EXPLICIT = 4
+};
+
+typedef struct {
+    int32_t block_points;       /* max points per rho-stage block (default 256) */
+    int32_t group_points;       /* target points per V-stage group (default 1024) */
+    size_t memory_limit_bytes;  /* 0 = no limit */
+    int32_t reserved[4];
+} CalchemVxcOptions;
+
+typedef struct {
+    int64_t n_points_in, n_points, n_blocks, n_groups, n_rho_tiles, n_v_pairs;
+    double flops_rho;                /* sum over rho blocks of 2 m_b^2 n_b, unpadded */
+    double flops_v;                  /* sum over V groups of 2 m_g^2 n_g, unpadded */
+    double flops_rho_executed_fp64;  /* with tile padding, as issued to MFMA */
+    double flops_rho_executed_fp32;
+    double flops_v_executed;
+    int32_t max_m_rho, max_m_v, rho_points_fp64, rho_points_fp32;
+    size_t device_bytes;
+    int64_t profiled_builds;         /* builds timed since profiling was switched on */
+    double ms_convert_and_rho;       /* summed HIP-event times of those builds */
+    double ms_vxc;
+    double ms_reduce_v;
+    double ms_reduce_en;
+} CalchemVxcStats;
+
+/* ---- Device queries: crates/calchem-cuda/src/lib.rs:469-489 (is_cuda_available,
+ *      CALCHEM_DISABLE_GPU) and cuda_kernels.cu exports
+ *      calchem_cuda_is_available / calchem_cuda_get_memory_info /
+ *      calchem_cuda_total_allocated_bytes.  CALCHEM_HIP_DISABLE=1 reports no device. */
+int calchem_hip_is_available(void);
+int calchem_hip_get_memory_info(size_t *free_bytes, size_t *total_bytes);
+size_t calchem_hip_total_allocated_bytes(void);
+
+/* ---- Reference-compatible GPU backend (functional on the host) ----
+ * Replaces calchem_cuda_vxc_upload_grid (cuda_kernels.cu:7354-7450; Rust
+ * binding crates/calchem-cuda/src/lib.rs:303-319, used by GpuVxcGrid::upload
+ * lib.rs:1195-1260).  The batch CSR is validated; points are re-blocked
+ * spatially inside the library. */
+int calchem_hip_vxc_upload_grid(const CalchemAoShell *shells, int n_shells, const CalchemAoPrim *prims,
+                                int n_prims, const CalchemAoComp *comps, int n_comps,
+                                const CalchemGridPt *pts, int n_pts, const int *cand_offsets,
+                                int n_cand_offsets, const int *cand_shells, int n_cand_shells, int n_ao,
+                                uint64_t grid_id, size_t memory_limit_bytes);
+/* Replaces calchem_cuda_vxc_eval_rho (cuda_kernels.cu:7452-7494; lib.rs:320-326,
+ * GpuVxcGrid::eval_rho_grad lib.rs:1262-1283): rho_grad[n_pts*4] =
+ * (rho, d rho/dx, d rho/dy, d rho/dz) per uploaded point for density P (n_ao^2). */
+int calchem_hip_vxc_eval_rho(uint64_t grid_id, const double *P, double *rho_grad, int n_pts, int n_ao);
+/* Replaces calchem_cuda_vxc_accum (cuda_kernels.cu:7496-7545; lib.rs:327-333,
+ * GpuVxcGrid::accumulate_vxc lib.rs:1285-1306): wv[n_pts*4] = (w v_rho_a / 2,
+ * w u_x, w u_y, w u_z) per point; V_out (n_ao^2) receives the full symmetric
+ * Vxc (its lower triangle is what the reference returns). */
+int calchem_hip_vxc_accum(uint64_t grid_id, const double *wv, double *V_out, int n_pts, int n_ao);
+/* Replaces calchem_cuda_vxc_release (cuda_kernels.cu:7547-7562; lib.rs:334,
+ * GpuVxcGrid::drop lib.rs:1184-1193).  Frees handles of either kind. */
+int calchem_hip_vxc_release(uint64_t grid_id);
+
+/* ---- Fused build (functional on the device) ----
+ * Replaces the per-geometry setup of RksVxcEngine::new_with_gpu
+ * (crates/calchem-dft/src/vxc.rs:429-595): xyzw[n_pts*4] = (x, y, z, weight)
+ * of MolecularGrid.points (grid.rs:33-51). */
+int calchem_hip_vxc_create(const CalchemAoShell *shells, int n_shells, const CalchemAoPrim *prims, int n_prims,
+                           const CalchemAoComp *comps, int n_comps, const double *xyzw, int n_pts, int n_ao,
+                           const CalchemVxcOptions *opts, uint64_t handle);
+/* Replaces RksVxcEngine::evaluate (vxc.rs:720-985) for a spin-summed density
+ * P (n_ao^2, host memory): V_out (n_ao^2, full symmetric), *exc = E_xc,
+ * *nelec = sum_g w_g rho_g (may be NULL).  Synchronous. */
+int calchem_hip_vxc_build(uint64_t handle, int xc, int precision, const double *P, double *V_out, double *exc,
+                          double *nelec);
+/* Same build on device pointers, enqueued on `stream` (a hipStream_t, NULL =
+ * default stream) without synchronising: d_exc_nelec[0] = E_xc,
+ * d_exc_nelec[1] = N.  Builds on one handle share its scratch: issue them on
+ * one stream. */
+int calchem_hip_vxc_build_device(uint64_t handle, int xc, int precision, const double *d_P, double *d_V,
+                                 double *d_exc_nelec, void *stream);
+/* Per-kernel HIP-event timing of subsequent builds (on = 1) and plan statistics;
+ * no reference counterpart (the reference prints CALCHEM_PROFILE stage timers,
+ * vxc.rs:571-578). */
+int calchem_hip_vxc_set_profiling(uint64_t handle, int on);
+int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats *out);
+
+/* ---- Molecular grid ----
+ * Replaces MolecularGrid::build / build_for_basis (grid.rs:614-634,732-828)
+ * and an explicit unpruned (n_rad x n_ang) grid.  Becke weights are computed
+ * on the GPU.  out[max_out*6] rows = (x, y, z, weight, raw_weight, host_atom),
+ * in the reference's atom / radial / angular order, weight > 1e-18 only.
+ * With out == NULL returns the point count; otherwise the count written, or
+ * -6 when max_out is too small. */
+long calchem_hip_grid_build(int n_atoms, const int *z, const double *xyz_bohr, int level, int n_rad, int n_ang,
+                            int radial_min, double *out, long max_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
