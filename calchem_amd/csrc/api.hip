@@ -654,6 +654,7 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats* out) {
     out->rho_points_fp64 = P.pts_fp64;
     out->rho_points_fp32 = P.pts_fp32;
     out->device_bytes = P.device_bytes();
+    out->lds_bytes_per_block = P.lds_limit;
     out->profiled_builds = P.prof.builds;
     out->ms_convert_and_rho = P.prof.ms[0];
     out->ms_vxc = P.prof.ms[1];

@@ -204,6 +204,11 @@ double ipow(double x, int n) {
     return r;
 }
 
+// The partition is evaluated without fused multiply-add so that its
+// threshold tests (1e-20 products, 1e-17 activity, w > 1e-18) see the same
+// roundings as the reference's separately rounded IEEE operations.
+#pragma clang fp contract(off)
+
 __device__ __forceinline__ double step3(double nu) {
     if (nu <= -1.0) return 1.0;
     if (nu >= 1.0) return 0.0;

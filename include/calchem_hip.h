@@ -89,6 +89,7 @@ typedef struct {
     double flops_v_executed;
     int32_t max_m_rho, max_m_v, rho_points_fp64, rho_points_fp32;
     size_t device_bytes;
+    size_t lds_bytes_per_block;      /* device limit used to size the rho tiles */
     int64_t profiled_builds;         /* builds timed since profiling was switched on */
     double ms_convert_and_rho;       /* summed HIP-event times of those builds */
     double ms_vxc;
