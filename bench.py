@@ -72,10 +72,12 @@ def cpu_baseline(basis, grid, func, P, target_s, threads):
     pts = grid.points
     n_batches = (pts.shape[0] + 127) // 128
     chunks = 8
-    per = 2
+    probe = max(threads, 16)
     t0 = time.perf_counter()
-    O.vxc_rks_cached_range(ob, pts, func, P, n_batches // 2, n_batches // 2 + per, threads)
-    est = (time.perf_counter() - t0) / per
+    for k in range(chunks):
+        s = int(k * n_batches / chunks)
+        O.vxc_rks_cached_range(ob, pts, func, P, s, min(s + probe // chunks + 1, n_batches), threads)
+    est = (time.perf_counter() - t0) / (chunks * (probe // chunks + 1))
     per = int(max(1, min(n_batches // chunks, target_s / max(est, 1e-6) / chunks)))
     starts = [int(k * n_batches / chunks) for k in range(chunks)]
     tot_t, tot_b, e_s, V_s, n_s = 0.0, 0, 0.0, None, 0.0

@@ -29,6 +29,8 @@ struct Mma;
 template <>
 struct Mma<double> {
     using acc = v4d;
+    // Result row of element v in lanes 16 q .. 16 q + 15.
+    __device__ static __forceinline__ int row(int q, int v) { return 4 * v + q; }
     __device__ static __forceinline__ acc zero() { return acc{0.0, 0.0, 0.0, 0.0}; }
     __device__ static __forceinline__ acc op(double a, double b, acc c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -37,6 +39,7 @@ struct Mma<double> {
 template <>
 struct Mma<float> {
     using acc = v4f;
+    __device__ static __forceinline__ int row(int q, int v) { return 4 * q + v; }
     __device__ static __forceinline__ acc zero() { return acc{0.f, 0.f, 0.f, 0.f}; }
     __device__ static __forceinline__ acc op(float a, float b, acc c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -44,8 +47,9 @@ struct Mma<float> {
 };
 
 // MFMA 16x16x4 operand layout (one value per lane): A[i][k] at lane i + 16 k,
-// B[k][j] at lane j + 16 k; the 16x16 result holds D[4 q + v][j] in element v
-// of lane j + 16 q.
+// B[k][j] at lane j + 16 k.  Element v of lane j + 16 q of the 16x16 result
+// holds D[Mma<T>::row(q, v)][j]: row 4 q + v for the f32 instruction, row
+// 4 v + q for the f64 one (one output row group per register).
 
 // ---------------------------------------------------------------------------
 // rho stage
@@ -109,10 +113,10 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
 #pragma unroll
             for (int h = 0; h < NH; ++h) acc[h] = M::op(av, sPhi[kk * PTS + h * 16 + col], acc[h]);
         }
-        // Epilogue: this lane holds T[i0 + 4 kq + v][h*16 + col].
+        // Epilogue: this lane holds T[i0 + row(kq, v)][h*16 + col].
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int mu = sAo[i0 + 4 * kq + v];
+            const int mu = sAo[i0 + M::row(kq, v)];
             if (mu < 0) continue;
             const DevAo ao = a.aos[mu];
 #pragma unroll
@@ -291,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
         for (int y = 0; y < 2; ++y)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int row = wr * 32 + x * 16 + kq * 4 + v;
+                const int row = wr * 32 + x * 16 + M::row(kq, v);
                 const int cj = wc * 32 + y * 16 + col;
                 out[row * kVTile + cj] = static_cast<double>(acc[x][y][v]);
             }
