@@ -1509,3 +1509,25 @@ int or_rho_points(const or_basis_t *b, long n_pts, const double *pts, const doub
     shells_free(&S);
     return 0;
 }
+
+/* Dense AO values and gradients at many points (test helper):            */
+/* val [n_pts*n_ao], grad [n_pts*n_ao*3], zero where the value rule drops. */
+int or_ao_dense(const or_basis_t *b, long n_pts, const double *pts, double *val, double *grad) {
+    shells_t S;
+    if (shells_build(b, &S) != 0) return -1;
+    int n = S.n_ao;
+    aoval_t *act = (aoval_t *)malloc(sizeof(aoval_t) * (size_t)(n + 1));
+    memset(val, 0, sizeof(double) * (size_t)n_pts * n);
+    memset(grad, 0, sizeof(double) * 3 * (size_t)n_pts * n);
+    for (long g = 0; g < n_pts; ++g) {
+        int na = eval_aos(&S, NULL, S.n_shells, pts + 4 * g, act);
+        for (int a = 0; a < na; ++a) {
+            size_t o = (size_t)g * n + act[a].mu;
+            val[o] = act[a].val;
+            for (int q = 0; q < 3; ++q) grad[3 * o + q] = act[a].g[q];
+        }
+    }
+    free(act);
+    shells_free(&S);
+    return 0;
+}

@@ -83,6 +83,9 @@ int or_vxc_uks_perpoint(const or_basis_t *b, long n_pts, const double *pts, int 
 int or_rho_points(const or_basis_t *b, long n_pts, const double *pts, const double *P, int n_threads,
                   double *out);
 
+/* Dense AO values [n_pts*n_ao] and gradients [n_pts*n_ao*3] (x, y, z, w rows). */
+int or_ao_dense(const or_basis_t *b, long n_pts, const double *pts, double *val, double *grad);
+
 /* Analytic overlap S (Obara-Saika 1D recursion), [n_ao*n_ao]. */
 int or_overlap(const or_basis_t *b, double *S);
 

@@ -61,6 +61,7 @@ def lib():
                                           C.POINTER(C.c_double)]
         L.or_overlap.argtypes = [C.POINTER(_Basis), _dp]
         L.or_rho_points.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, C.c_int, _dp]
+        L.or_ao_dense.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -199,3 +200,14 @@ def rho_points(ob: OracleBasis, grid_rows, P, threads=None):
     if rc != 0:
         raise RuntimeError(rc)
     return out.reshape(-1, 4)
+
+
+def ao_dense(ob: OracleBasis, grid_rows):
+    pts = _pts4(grid_rows)
+    n = pts.shape[0]
+    val = np.zeros(n * ob.n_ao)
+    grad = np.zeros(n * ob.n_ao * 3)
+    rc = lib().or_ao_dense(C.byref(ob.struct), n, pts.ravel(), val, grad)
+    if rc != 0:
+        raise RuntimeError(rc)
+    return val.reshape(n, ob.n_ao), grad.reshape(n, ob.n_ao, 3)
