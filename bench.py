@@ -108,6 +108,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block-points", type=int, default=0)
     ap.add_argument("--group-points", type=int, default=0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -141,6 +143,14 @@ def main() -> int:
 
     n = basis.n_ao
     stream = hr.Stream()
+    if args.copy_calibration_mib > 0:
+        nb = args.copy_calibration_mib << 20
+        src, dst = hr.DeviceArray(nb), hr.DeviceArray(nb)
+        hr.hip().hipMemcpy(dst.ptr, src.ptr, nb, 3)  # device to device
+        hr.device_synchronize()
+        log(f"copy calibration: {args.copy_calibration_mib} MiB device-to-device")
+        src.free()
+        dst.free()
     dP = hr.DeviceArray.from_numpy(P)
     dV = hr.DeviceArray(n * n * 8)
     den = hr.DeviceArray(16)

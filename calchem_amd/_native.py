@@ -28,7 +28,8 @@ class CalchemHipError(RuntimeError):
 
 class VxcOptions(C.Structure):
     _fields_ = [("block_points", C.c_int32), ("group_points", C.c_int32),
-                ("memory_limit_bytes", C.c_size_t), ("reserved", C.c_int32 * 4)]
+                ("memory_limit_bytes", C.c_size_t), ("mixed_exact_threshold", C.c_double),
+                ("reserved", C.c_int32 * 4)]
 
 
 class VxcStats(C.Structure):
@@ -41,7 +42,8 @@ class VxcStats(C.Structure):
                 ("rho_points_fp32", C.c_int32), ("device_bytes", C.c_size_t),
                 ("lds_bytes_per_block", C.c_size_t),
                 ("profiled_builds", C.c_int64), ("ms_convert_and_rho", C.c_double),
-                ("ms_vxc", C.c_double), ("ms_reduce_v", C.c_double), ("ms_reduce_en", C.c_double)]
+                ("ms_vxc", C.c_double), ("ms_reduce_v", C.c_double), ("ms_reduce_en", C.c_double),
+                ("flops_rho_fp64_rows", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -60,4 +60,88 @@ __device__ __forceinline__ AoVal ao_eval(const DevShell* __restrict__ shells, co
     return o;
 }
 
+// The same evaluation split in two: the radial sums of one shell at one point
+// (zero when the shell is cut off or both sums fall below 1e-18), then any
+// Cartesian component of that shell from them.
+__device__ __forceinline__ double2 shell_radial(const DevShell& s, const double2* __restrict__ prims, double x,
+                                                double y, double z) {
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double r0 = 0.0, r1 = 0.0;
+    if (r2 <= s.r2cut) {
+        for (int p = 0; p < s.n_prims; ++p) {
+            const double2 pr = prims[s.prim_start + p];
+            const double e = pr.y * exp(-pr.x * r2);
+            r0 += e;
+            r1 -= 2.0 * pr.x * e;
+        }
+        if (fabs(r0) < 1e-18 && fabs(r1) < 1e-18) r0 = r1 = 0.0;
+    }
+    return make_double2(r0, r1);
+}
+
+__device__ __forceinline__ AoVal ao_from_radial(const DevShell& s, const DevAo& ao, double x, double y, double z,
+                                                double2 rr) {
+    AoVal o{0.0, 0.0, 0.0, 0.0};
+    if (rr.x == 0.0 && rr.y == 0.0) return o;
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    double px, dpx, py, dpy, pz, dpz;
+    mono_d(dx, ao.lx, px, dpx);
+    mono_d(dy, ao.ly, py, dpy);
+    mono_d(dz, ao.lz, pz, dpz);
+    const double m = px * py * pz;
+    const double sc = ao.scale;
+    const double v = sc * m * rr.x;
+    const double gx = sc * (dpx * py * pz * rr.x + m * dx * rr.y);
+    const double gy = sc * (px * dpy * pz * rr.x + m * dy * rr.y);
+    const double gz = sc * (px * py * dpz * rr.x + m * dz * rr.y);
+    if (fabs(v) > 1e-16 || fabs(gx) > 1e-16 || fabs(gy) > 1e-16 || fabs(gz) > 1e-16) {
+        o.v = v;
+        o.gx = gx;
+        o.gy = gy;
+        o.gz = gz;
+    }
+    return o;
+}
+
+// ao_from_radial for a row held in LDS.
+__device__ __forceinline__ AoVal ao_from_radial_row(const DevShell& s, const RowL& r, double x, double y, double z,
+                                                    double2 rr) {
+    DevAo ao;
+    ao.scale = r.scale;
+    ao.shell = r.sh;
+    ao.lx = r.lmn & 3;
+    ao.ly = (r.lmn >> 2) & 3;
+    ao.lz = (r.lmn >> 4) & 3;
+    return ao_from_radial(s, ao, x, y, z, rr);
+}
+
+// Stage the rows, shells (with block-local primitive offsets) and primitives
+// of a block or group into LDS.  rows: (global AO, local shell); shl:
+// (global shell, local primitive offset).
+__device__ __forceinline__ void stage_meta(const int2* __restrict__ rows, int m, int mp,
+                                           const int2* __restrict__ shl, int nsh,
+                                           const DevShell* __restrict__ shells, const double2* __restrict__ prims,
+                                           const DevAo* __restrict__ aos, RowL* sRow, DevShell* sSh,
+                                           double2* sPrim, int tid, int nthreads) {
+    for (int r = tid; r < mp; r += nthreads) {
+        RowL o{0.0, 0, -1};
+        if (r < m) {
+            const int2 e = rows[r];
+            const DevAo ao = aos[e.x];
+            o.scale = ao.scale;
+            o.lmn = ao.lx | (ao.ly << 2) | (ao.lz << 4);
+            o.sh = e.y;
+        }
+        sRow[r] = o;
+    }
+    for (int s = tid; s < nsh; s += nthreads) {
+        const int2 e = shl[s];
+        DevShell d = shells[e.x];
+        for (int p = 0; p < d.n_prims; ++p) sPrim[e.y + p] = prims[d.prim_start + p];
+        d.prim_start = e.y;
+        sSh[s] = d;
+    }
+}
+
 }  // namespace chx

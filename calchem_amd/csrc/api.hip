@@ -69,11 +69,18 @@ struct Plan {
     int n_blocks = 0, n_groups = 0, n_pairs = 0;
     int n_tiles[2] = {0, 0};  // rho tiles of 32 / 16 points
     int max_mpad_rho = 0, max_m_v = 0;
-    int pts_fp64 = 32, pts_fp32 = 32;
-    double flops_rho = 0, flops_v = 0, flops_rho_pad[2] = {0, 0}, flops_v_pad = 0;
+    // Launch classes per granularity (32 / 16 points) and the granularity
+    // each class uses per mode (FP64 / mixed); 0 = does not fit in LDS.
+    std::vector<RhoClass> classes[2];
+    std::vector<int> class_pts[2];
+    double flops_rho = 0, flops_v = 0, flops_rho_pad[2] = {0, 0}, flops_v_pad = 0, flops_rho_fp64_rows = 0;
+    double mixed_threshold = 0.25;
     size_t lds_limit = 0;
-    DevBuf shells, prims, aos, pts, perm, blocks, block_ao, tiles32, tiles16, groups, group_ao, pairs, loc,
-        scratch, wv, wv_caller, epart, npart, en, P64, P32, V, rho;
+    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, n_xc_blocks = 0;
+    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, tiles32, tiles16, groups, group_rows,
+        group_sh, group_list, pairs, loc, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, P32,
+        V, rho, rho_s, psub64, psub32;
+    long long psub_elems = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
     Profile prof;
@@ -82,9 +89,10 @@ struct Plan {
         if (stream) hipStreamDestroy(stream);
     }
     size_t device_bytes() const {
-        const DevBuf* b[] = {&shells, &prims, &aos,     &pts,   &perm,  &blocks,    &block_ao, &tiles32,
-                             &tiles16, &groups, &group_ao, &pairs, &loc,   &scratch,   &wv,       &wv_caller,
-                             &epart,  &npart,  &en,      &P64,   &P32,   &V,         &rho};
+        const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
+                             &block_sh, &tiles32,   &tiles16,  &groups, &group_rows, &group_sh, &group_list,
+                             &pairs,    &loc,       &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
+                             &epart,    &npart,     &en,       &P64,   &P32,       &V,        &rho, &rho_s, &psub64, &psub32};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
         return s;
@@ -190,6 +198,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     const int nb_rho = (opt && opt->block_points > 0) ? round_up(opt->block_points, 32) : 256;
     const int nb_v = (opt && opt->group_points > 0) ? opt->group_points : 1024;
     const size_t mem_limit = opt ? opt->memory_limit_bytes : 0;
+    if (opt && opt->mixed_exact_threshold > 0.0) P.mixed_threshold = opt->mixed_exact_threshold;
 
     // Device shell / AO tables.
     std::vector<DevShell> hs(n_shells);
@@ -270,84 +279,169 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     bool ok = upload(P.shells, hs) && upload(P.prims, hp) && upload(P.aos, hao) && upload(P.pts, spts);
     if (!ok) return kAllocFail;
     std::vector<unsigned char> flag(cand_ao.size(), 0);
+    std::vector<double> maxabs(cand_ao.size(), 0.0);
     {
-        DevBuf dca, dcr, dfl;
-        if (!upload(dca, cand_ao) || !upload(dcr, cand_rng) || !dfl.alloc(flag.size() + 1)) return kAllocFail;
+        DevBuf dca, dcr, dfl, dmx;
+        if (!upload(dca, cand_ao) || !upload(dcr, cand_rng) || !dfl.alloc(flag.size() + 1) ||
+            !dmx.alloc(sizeof(double) * (maxabs.size() + 1)))
+            return kAllocFail;
         SigArgs sa{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-                   dca.as<int>(), dcr.as<int2>(), dfl.as<unsigned char>(), (long long)cand_ao.size()};
+                   dca.as<int>(), dcr.as<int2>(), dfl.as<unsigned char>(), dmx.as<double>(),
+                   (long long)cand_ao.size()};
         if (launch_significance(sa, 0) != hipSuccess) return kLaunchFail;
-        if (!flag.empty() && hipMemcpy(flag.data(), dfl.p, flag.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        if (!flag.empty() &&
+            (hipMemcpy(flag.data(), dfl.p, flag.size(), hipMemcpyDeviceToHost) != hipSuccess ||
+             hipMemcpy(maxabs.data(), dmx.p, sizeof(double) * maxabs.size(), hipMemcpyDeviceToHost) != hipSuccess))
             return kLaunchFail;
     }
 
-    // rho blocks, tiles and their AO lists.
+    // rho blocks: rows are the AOs that survive somewhere in the block, those
+    // with max |phi| above the mixed threshold first (each part ascending);
+    // each row names its shell in the block's shell list.
     std::vector<RhoBlock> hb(P.n_blocks);
-    std::vector<int> hbao;
+    std::vector<int2> hrows;
+    std::vector<int2> hsh;  // (global shell, block-local primitive offset)
     std::vector<std::vector<int>> blist(P.n_blocks);
-    std::vector<RhoTile> t32, t16;
     for (int b = 0; b < P.n_blocks; ++b) {
-        for (int k = cand_begin[b]; k < cand_begin[b + 1]; ++k)
-            if (flag[k]) blist[b].push_back(cand_ao[k]);
+        std::vector<int> big, small;
+        for (int k = cand_begin[b]; k < cand_begin[b + 1]; ++k) {
+            if (!flag[k]) continue;
+            blist[b].push_back(cand_ao[k]);
+            (maxabs[k] > P.mixed_threshold ? big : small).push_back(cand_ao[k]);
+        }
         std::sort(blist[b].begin(), blist[b].end());
         const int m = static_cast<int>(blist[b].size());
-        const int mp = round_up(std::max(m, 1), 32);
-        hb[b] = RhoBlock{static_cast<int>(hbao.size()), m, mp, 0};
-        hbao.insert(hbao.end(), blist[b].begin(), blist[b].end());
-        hbao.resize(hbao.size() + (mp - m), -1);
+        const int mp = round_up(std::max(m, 1), 16);
+        std::vector<int> rows = big;
+        rows.insert(rows.end(), small.begin(), small.end());
+        RhoBlock B{};
+        B.ao_begin = static_cast<int>(hrows.size());
+        B.m = m;
+        B.m_pad = mp;
+        B.m64 = std::min(round_up(static_cast<int>(big.size()), 4), mp);
+        B.sh_begin = static_cast<int>(hsh.size());
+        std::vector<int> local(n_shells, -1);
+        for (int mu : rows) {
+            const int s = hao[mu].shell;
+            if (local[s] < 0) {
+                local[s] = static_cast<int>(hsh.size()) - B.sh_begin;
+                hsh.push_back(make_int2(s, B.n_prim));
+                B.n_prim += hs[s].n_prims;
+            }
+            hrows.push_back(make_int2(mu, local[s]));
+        }
+        B.n_sh = static_cast<int>(hsh.size()) - B.sh_begin;
+        hrows.resize(hrows.size() + (mp - m), make_int2(-1, 0));
+        B.psub = P.psub_elems;
+        P.psub_elems += (long long)mp * mp;
+        hb[b] = B;
         P.max_mpad_rho = std::max(P.max_mpad_rho, mp);
         const int nb = leaves[b].second - leaves[b].first;
         P.flops_rho += 2.0 * m * (double)m * nb;
-        if (m == 0) continue;
-        for (int o = 0; o < nb; o += 32) {
-            t32.push_back(RhoTile{b, leaves[b].first + o, std::min(32, nb - o), 0});
-            P.flops_rho_pad[0] += 2.0 * mp * (double)mp * 32;
-        }
-        for (int o = 0; o < nb; o += 16) {
-            t16.push_back(RhoTile{b, leaves[b].first + o, std::min(16, nb - o), 0});
-            P.flops_rho_pad[1] += 2.0 * mp * (double)mp * 16;
-        }
+        P.flops_rho_fp64_rows += 2.0 * m * (double)big.size() * nb;
     }
-    P.n_tiles[0] = static_cast<int>(t32.size());
-    P.n_tiles[1] = static_cast<int>(t16.size());
+    // Tiles of 32 and of 16 points, grouped into launch classes by m_pad so
+    // each launch sizes its LDS for similar blocks.
+    std::vector<RhoTile> tl[2];
+    for (int g = 0; g < 2; ++g) {
+        const int pts = g == 0 ? 32 : 16;
+        const int ncls = (P.max_mpad_rho + 63) / 64;
+        std::vector<std::vector<RhoTile>> bycls(std::max(ncls, 1));
+        std::vector<RhoClass> cls(std::max(ncls, 1), RhoClass{0, 0, 0, 0, 0, 0});
+        for (int b = 0; b < P.n_blocks; ++b) {
+            if (hb[b].m == 0) continue;
+            const int c = (hb[b].m_pad - 1) / 64;
+            const int nb = leaves[b].second - leaves[b].first;
+            for (int o = 0; o < nb; o += pts) {
+                bycls[c].push_back(RhoTile{b, leaves[b].first + o, std::min(pts, nb - o), 0});
+                P.flops_rho_pad[g] += 2.0 * hb[b].m_pad * (double)hb[b].m_pad * pts;
+            }
+            cls[c].max_mpad = std::max(cls[c].max_mpad, hb[b].m_pad);
+            cls[c].max_nsh = std::max(cls[c].max_nsh, hb[b].n_sh);
+            cls[c].max_m64 = std::max(cls[c].max_m64, hb[b].m64);
+            cls[c].max_prim = std::max(cls[c].max_prim, hb[b].n_prim);
+        }
+        for (size_t c = 0; c < bycls.size(); ++c) {
+            if (bycls[c].empty()) continue;
+            cls[c].tile_begin = static_cast<int>(tl[g].size());
+            cls[c].n_tiles = static_cast<int>(bycls[c].size());
+            tl[g].insert(tl[g].end(), bycls[c].begin(), bycls[c].end());
+            P.classes[g].push_back(cls[c]);
+        }
+        P.n_tiles[g] = static_cast<int>(tl[g].size());
+    }
+    std::vector<RhoTile>& t32 = tl[0];
+    std::vector<RhoTile>& t16 = tl[1];
 
-    // V groups: consecutive blocks (k-d order) merged up to nb_v points; the
-    // group's AO list is the union of its blocks' lists.
+    // V groups: consecutive blocks (k-d order) merged up to nb_v points and,
+    // for the register-resident kernel, up to 272 rows; the group's rows are
+    // the union of its blocks' AOs, ascending.  A single block wider than that
+    // goes to the tile-pair kernel.
     std::vector<VGroup> hg;
-    std::vector<int> hgao;
+    std::vector<int2> hgrows, hgsh;  // hgsh: (global shell, group-local primitive offset)
+    std::vector<int> hglist;
     std::vector<VPair> hpairs;
     std::vector<int> hloc;
     long long scratch = 0;
-    for (int b = 0; b < P.n_blocks;) {
-        int e = b, npt = 0;
-        while (e < P.n_blocks && (npt == 0 || npt + (leaves[e].second - leaves[e].first) <= nb_v)) {
-            npt += leaves[e].second - leaves[e].first;
-            ++e;
-        }
+    const int max_rows = kGroupMaxTiles * 16;
+    auto union_rows = [&](int b, int e) {
         std::vector<int> u;
         for (int k = b; k < e; ++k) u.insert(u.end(), blist[k].begin(), blist[k].end());
         std::sort(u.begin(), u.end());
         u.erase(std::unique(u.begin(), u.end()), u.end());
+        return u;
+    };
+    for (int b = 0; b < P.n_blocks;) {
+        int e = b + 1, npt = leaves[b].second - leaves[b].first;
+        std::vector<int> u = union_rows(b, e);
+        while (e < P.n_blocks && npt + (leaves[e].second - leaves[e].first) <= nb_v) {
+            std::vector<int> u2 = union_rows(b, e + 1);
+            if ((int)u2.size() > max_rows && (int)u.size() <= max_rows) break;
+            u.swap(u2);
+            npt += leaves[e].second - leaves[e].first;
+            ++e;
+        }
         const int m = static_cast<int>(u.size());
         if (m > 0) {
-            VGroup G;
-            G.ao_begin = static_cast<int>(hgao.size());
+            VGroup G{};
+            const int gidx = static_cast<int>(hg.size());
+            G.ao_begin = static_cast<int>(hgrows.size());
             G.m = m;
-            G.n_tiles64 = (m + kVTile - 1) / kVTile;
-            G.m_pad = G.n_tiles64 * kVTile;
             G.pt_begin = leaves[b].first;
             G.n_pts = npt;
             G.scratch_off = scratch;
-            const int gidx = static_cast<int>(hg.size());
-            for (int i = 0; i < G.n_tiles64; ++i)
-                for (int j = 0; j <= i; ++j) hpairs.push_back(VPair{gidx, i, j, 0});
-            scratch += (long long)G.n_tiles64 * (G.n_tiles64 + 1) / 2 * kVTile * kVTile;
-            hgao.insert(hgao.end(), u.begin(), u.end());
+            G.sh_begin = static_cast<int>(hgsh.size());
+            std::vector<int> local(n_shells, -1);
+            for (int mu : u) {
+                const int sidx = hao[mu].shell;
+                if (local[sidx] < 0) {
+                    local[sidx] = static_cast<int>(hgsh.size()) - G.sh_begin;
+                    hgsh.push_back(make_int2(sidx, G.n_prim));
+                    G.n_prim += hs[sidx].n_prims;
+                }
+                hgrows.push_back(make_int2(mu, local[sidx]));
+            }
+            G.n_sh = static_cast<int>(hgsh.size()) - G.sh_begin;
+            if (m <= max_rows) {
+                G.tile_shift = 4;
+                G.n_tiles = (m + 15) / 16;
+                hglist.push_back(gidx);
+                P.max_group_nsh = std::max(P.max_group_nsh, G.n_sh);
+                P.max_group_nprim = std::max(P.max_group_nprim, G.n_prim);
+                P.flops_v_pad += 2.0 * G.n_tiles * (G.n_tiles + 1) * 256.0 * (double)round_up(npt, kVPts);
+            } else {
+                G.tile_shift = 6;
+                G.n_tiles = (m + kVTile - 1) / kVTile;
+                for (int i = 0; i < G.n_tiles; ++i)
+                    for (int j = 0; j <= i; ++j) hpairs.push_back(VPair{gidx, i, j, 0});
+                P.flops_v_pad += 2.0 * G.n_tiles * (G.n_tiles + 1) * (double)(kVTile * kVTile) *
+                                 (double)round_up(npt, kVPts);
+            }
+            scratch += (long long)G.n_tiles * (G.n_tiles + 1) / 2 << (2 * G.tile_shift);
             const size_t lo = hloc.size();
             hloc.resize(lo + n_ao, -1);
             for (int i = 0; i < m; ++i) hloc[lo + u[i]] = i;
             P.flops_v += 2.0 * m * (double)m * npt;
-            P.flops_v_pad += 2.0 * G.n_tiles64 * (G.n_tiles64 + 1) * (double)(kVTile * kVTile) *
-                             (double)round_up(npt, kVPts);
             P.max_m_v = std::max(P.max_m_v, m);
             hg.push_back(G);
         }
@@ -355,36 +449,60 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     }
     P.n_groups = static_cast<int>(hg.size());
     P.n_pairs = static_cast<int>(hpairs.size());
+    P.n_group_list = static_cast<int>(hglist.size());
+    // CSR: for each AO the groups holding it, ascending.
+    std::vector<int> hgb(n_ao + 1, 0), hgg;
+    for (int g = 0; g < P.n_groups; ++g)
+        for (int i = 0; i < hg[g].m; ++i) hgb[hgrows[hg[g].ao_begin + i].x + 1]++;
+    for (int mu = 0; mu < n_ao; ++mu) hgb[mu + 1] += hgb[mu];
+    hgg.resize(hgb[n_ao]);
+    {
+        std::vector<int> fill(hgb.begin(), hgb.end() - 1);
+        for (int g = 0; g < P.n_groups; ++g)
+            for (int i = 0; i < hg[g].m; ++i) hgg[fill[hgrows[hg[g].ao_begin + i].x]++] = g;
+    }
 
-    // LDS budget of the rho kernel decides 32- or 16-point tiles per precision.
+    // LDS budget: per class and mode, 32-point tiles when they fit, else 16.
     int lds_max = 0;
-    hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device);
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device) != hipSuccess)
+        lds_max = 65536;
     P.lds_limit = static_cast<size_t>(std::max(lds_max, 65536));
-    auto pick = [&](int bytes) {
-        if (rho_lds_bytes(bytes, 32, P.max_mpad_rho) <= P.lds_limit) return 32;
-        if (rho_lds_bytes(bytes, 16, P.max_mpad_rho) <= P.lds_limit) return 16;
-        return 0;
-    };
-    P.pts_fp64 = pick(8);
-    P.pts_fp32 = pick(4);
+    // class_pts[mode][class]: classes are indexed alike in both granularities
+    // (same blocks), so the 32-point class list drives the choice.
+    for (int mode = 0; mode < 2; ++mode) {
+        P.class_pts[mode].assign(P.classes[0].size(), 0);
+        for (size_t c = 0; c < P.classes[0].size(); ++c) {
+            if (rho_lds_bytes(mode, 32, P.classes[0][c]) <= P.lds_limit) P.class_pts[mode][c] = 32;
+            else if (rho_lds_bytes(mode, 16, P.classes[1][c]) <= P.lds_limit) P.class_pts[mode][c] = 16;
+        }
+    }
 
+    if (P.n_group_list > 0 && vgroup_lds_bytes(false, P.max_group_nsh, P.max_group_nprim) > P.lds_limit) return kLimit;
     const size_t need = sizeof(double) * (size_t)scratch + sizeof(int) * hloc.size() +
                         sizeof(double) * (size_t)n_ao * n_ao * 2 + sizeof(float) * (size_t)n_ao * n_ao +
-                        sizeof(double4) * (size_t)(2 * P.n_pts + n_pts);
+                        sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
     if (mem_limit > 0 && need > mem_limit) return kLimit;
 
-    const int ntile_max = std::max(P.n_tiles[0], P.n_tiles[1]);
+    P.n_xc_blocks = (P.n_pts + kThreads - 1) / kThreads;
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
-    ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_ao, hbao) && upload(P.tiles32, t32) &&
-         upload(P.tiles16, t16) && upload(P.groups, hg) && upload(P.group_ao, hgao) && upload(P.pairs, hpairs) &&
-         upload(P.loc, hloc) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
+    ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
+         upload(P.tiles32, t32) &&
+         upload(P.tiles16, t16) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
+         upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
+         upload(P.loc, hloc) && upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
-         P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (ntile_max + 1)) &&
-         P.npart.alloc(sizeof(double) * (ntile_max + 1)) && P.en.alloc(sizeof(double) * 2) &&
+         P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) &&
+         P.npart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) && P.en.alloc(sizeof(double) * 2) &&
          P.P64.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.P32.alloc(sizeof(float) * (size_t)n_ao * n_ao) &&
-         P.V.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.rho.alloc(sizeof(double4) * (size_t)n_pts);
+         P.V.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.rho.alloc(sizeof(double4) * (size_t)n_pts) &&
+         P.rho_s.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
+         P.psub64.alloc(sizeof(double) * (size_t)std::max<long long>(P.psub_elems, 1)) &&
+         P.psub32.alloc(sizeof(float) * (size_t)std::max<long long>(P.psub_elems, 1));
     if (!ok) return kAllocFail;
+    // Points of blocks with no significant AO are never written by the rho
+    // kernel: their sorted rho stays zero for the life of the plan.
+    if (hipMemset(P.rho_s.p, 0, P.rho_s.bytes) != hipSuccess) return kLaunchFail;
     if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) return kNoDevice;
     if (hipDeviceSynchronize() != hipSuccess) return kLaunchFail;
     std::lock_guard<std::mutex> lk(g_table_mu);
@@ -415,45 +533,69 @@ void resolve_profile(Plan& P) {
     ev.clear();
 }
 
+// Rho stage: gather the density blocks, then every launch class with its own
+// LDS budget.  rho goes to P.rho_s per sorted point, or to P.rho per caller
+// point when caller_order (points dropped at plan time keep what P.rho held).
+int run_rho(Plan& P, int mode, bool caller_order, const double* dP, hipStream_t s) {
+    GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), dP, P.n_ao, P.psub64.as<double>(),
+                  mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
+    if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
+    const double* dP64 = P.psub64.as<double>();
+    const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
+    for (size_t c = 0; c < P.classes[0].size(); ++c) {
+        const int pts = P.class_pts[mode][c];
+        if (pts == 0) return kLimit;
+        const int g = pts == 32 ? 0 : 1;
+        RhoClass cl = P.classes[g][c];
+        RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
+                   g == 0 ? P.tiles32.as<RhoTile>() : P.tiles16.as<RhoTile>(), P.blocks.as<RhoBlock>(),
+                   P.block_rows.as<int2>(), P.block_sh.as<int2>(), dP64, dPS,
+                   caller_order ? P.rho.as<double4>() : P.rho_s.as<double4>(),
+                   P.perm.as<int>(), caller_order ? 1 : 0};
+        if (launch_rho(ra, mode, pts, cl, s) != hipSuccess) return kLaunchFail;
+    }
+    return kOk;
+}
+
+// V stage: register-resident kernel over its groups, tile-pair kernel over
+// the rest, then the deterministic reduction into the full symmetric V.
+int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s) {
+    VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
+             P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
+             P.group_sh.as<int2>(), wv, P.perm.as<int>(), wv_perm, P.scratch.as<double>()};
+    if (launch_vxc_group(va, f32, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) != hipSuccess)
+        return kLaunchFail;
+    if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
+    mark(P, s);
+    const long long nn = (long long)P.n_ao * P.n_ao;
+    ReduceArgs rd{P.loc.as<int>(), P.ao_gbegin.as<int>(), P.ao_groups.as<int>(), P.groups.as<VGroup>(),
+                  P.n_groups, P.n_ao, P.scratch.as<double>(), dV};
+    if (P.n_groups == 0) {
+        if (launch_fill_zero(dV, nn, s) != hipSuccess) return kLaunchFail;
+    } else if (launch_reduce_v(rd, s) != hipSuccess) {
+        return kLaunchFail;
+    }
+    return kOk;
+}
+
 // Fused RKS build on device pointers, asynchronous on stream s.
 int run_build(Plan& P, int func, int prec, const double* dP, double* dV, double* den, hipStream_t s) {
     if (func < 0 || func > 5) return kUnsupported;
     if (prec != CALCHEM_PREC_FP64 && prec != CALCHEM_PREC_FP32) return kUnsupported;
     if (!P.has_weights) return kBadArgs;
     const bool f32 = prec == CALCHEM_PREC_FP32;
-    const int pts = f32 ? P.pts_fp32 : P.pts_fp64;
-    if (pts == 0) return kLimit;
-    const long long nn = (long long)P.n_ao * P.n_ao;
     mark(P, s);
-    const void* Pk = dP;
-    if (f32) {
-        if (launch_to_float(dP, P.P32.as<float>(), nn, s) != hipSuccess) return kLaunchFail;
-        Pk = P.P32.p;
-    }
-    const int ti = pts == 32 ? 0 : 1;
-    RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-               ti == 0 ? P.tiles32.as<RhoTile>() : P.tiles16.as<RhoTile>(), P.blocks.as<RhoBlock>(),
-               P.block_ao.as<int>(), Pk, P.n_ao, func, P.wv.as<double4>(), nullptr, P.perm.as<int>(),
-               P.epart.as<double>(), P.npart.as<double>()};
-    if (launch_rho(ra, f32, pts, true, P.n_tiles[ti], P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
+    const int rc = run_rho(P, f32 ? kRhoMixed : kRhoFp64, false, dP, s);
+    if (rc != kOk) return rc;
+    XcArgs xa{P.rho_s.as<double4>(), P.pts.as<double4>(), P.n_pts, func, P.wv.as<double4>(), P.epart.as<double>(),
+              P.npart.as<double>()};
+    if (launch_xc(xa, s) != hipSuccess) return kLaunchFail;
     mark(P, s);
-    VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-             P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_ao.as<int>(), P.wv.as<double4>(),
-             P.perm.as<int>(), 0, P.scratch.as<double>()};
-    if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
+    const int rv = run_v(P, f32, P.wv.as<double4>(), 0, dV, s);
+    if (rv != kOk) return rv;
     mark(P, s);
-    ReduceArgs rd{P.loc.as<int>(), P.groups.as<VGroup>(), P.n_groups, P.n_ao, P.scratch.as<double>(), dV};
-    if (P.n_groups == 0) {
-        if (hipMemsetAsync(dV, 0, sizeof(double) * nn, s) != hipSuccess) return kLaunchFail;
-    } else if (launch_reduce_v(rd, s) != hipSuccess) {
+    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
         return kLaunchFail;
-    }
-    mark(P, s);
-    if (P.n_tiles[ti] == 0) {
-        if (hipMemsetAsync(den, 0, sizeof(double) * 2, s) != hipSuccess) return kLaunchFail;
-    } else if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_tiles[ti], den, s) != hipSuccess) {
-        return kLaunchFail;
-    }
     mark(P, s);
     return kOk;
 }
@@ -517,15 +659,9 @@ int calchem_hip_vxc_eval_rho(uint64_t grid_id, const double* P_mat, double* rho_
     const size_t nn = (size_t)n_ao * n_ao;
     hipStream_t s = P.stream;
     if (hipMemcpyAsync(P.P64.p, P_mat, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess) return kLaunchFail;
-    if (hipMemsetAsync(P.rho.p, 0, sizeof(double4) * (size_t)n_pts, s) != hipSuccess) return kLaunchFail;
-    const int pts = P.pts_fp64;
-    if (pts == 0) return kLimit;
-    const int ti = pts == 32 ? 0 : 1;
-    RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-               ti == 0 ? P.tiles32.as<RhoTile>() : P.tiles16.as<RhoTile>(), P.blocks.as<RhoBlock>(),
-               P.block_ao.as<int>(), P.P64.p, n_ao, -1, nullptr, P.rho.as<double4>(), P.perm.as<int>(),
-               P.epart.as<double>(), P.npart.as<double>()};
-    if (launch_rho(ra, false, pts, false, P.n_tiles[ti], P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
+    if (launch_fill_zero(P.rho.as<double>(), 4LL * n_pts, s) != hipSuccess) return kLaunchFail;
+    const int rc = run_rho(P, kRhoFp64, true, P.P64.as<double>(), s);
+    if (rc != kOk) return rc;
     if (hipMemcpyAsync(rho_grad, P.rho.p, sizeof(double4) * (size_t)n_pts, hipMemcpyDeviceToHost, s) != hipSuccess)
         return kLaunchFail;
     return hipStreamSynchronize(s) == hipSuccess ? kOk : kLaunchFail;
@@ -543,17 +679,8 @@ int calchem_hip_vxc_accum(uint64_t grid_id, const double* wv_in, double* vxc_out
     const size_t nn = (size_t)n_ao * n_ao;
     if (hipMemcpyAsync(P.wv_caller.p, wv_in, sizeof(double4) * (size_t)n_pts, hipMemcpyHostToDevice, s) != hipSuccess)
         return kLaunchFail;
-    VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-             P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_ao.as<int>(), P.wv_caller.as<double4>(),
-             P.perm.as<int>(), 1, P.scratch.as<double>()};
-    if (launch_vxc(va, false, P.n_pairs, s) != hipSuccess) return kLaunchFail;
-    ReduceArgs rd{P.loc.as<int>(), P.groups.as<VGroup>(), P.n_groups, n_ao, P.scratch.as<double>(),
-                  P.V.as<double>()};
-    if (P.n_groups == 0) {
-        if (hipMemsetAsync(P.V.p, 0, sizeof(double) * nn, s) != hipSuccess) return kLaunchFail;
-    } else if (launch_reduce_v(rd, s) != hipSuccess) {
-        return kLaunchFail;
-    }
+    const int rv = run_v(P, false, P.wv_caller.as<double4>(), 1, P.V.as<double>(), s);
+    if (rv != kOk) return rv;
     if (hipMemcpyAsync(vxc_out, P.V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s) != hipSuccess)
         return kLaunchFail;
     return hipStreamSynchronize(s) == hipSuccess ? kOk : kLaunchFail;
@@ -646,13 +773,26 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats* out) {
     out->n_v_pairs = P.n_pairs;
     out->flops_rho = P.flops_rho;
     out->flops_v = P.flops_v;
-    out->flops_rho_executed_fp64 = P.flops_rho_pad[P.pts_fp64 == 16 ? 1 : 0];
-    out->flops_rho_executed_fp32 = P.flops_rho_pad[P.pts_fp32 == 16 ? 1 : 0];
+    double ex[2] = {0.0, 0.0};
+    int pts_used[2] = {0, 0};
+    for (int mode = 0; mode < 2; ++mode)
+        for (size_t c = 0; c < P.classes[0].size(); ++c) {
+            const int pts = P.class_pts[mode][c];
+            if (pts == 0) continue;
+            pts_used[mode] = std::max(pts_used[mode], pts);
+            const RhoClass& cl = P.classes[pts == 32 ? 0 : 1][c];
+            (void)cl;
+        }
+    ex[0] = P.flops_rho_pad[pts_used[0] == 16 ? 1 : 0];
+    ex[1] = P.flops_rho_pad[pts_used[1] == 16 ? 1 : 0];
+    out->flops_rho_executed_fp64 = ex[0];
+    out->flops_rho_executed_fp32 = ex[1];
     out->flops_v_executed = P.flops_v_pad;
     out->max_m_rho = P.max_mpad_rho;
     out->max_m_v = P.max_m_v;
-    out->rho_points_fp64 = P.pts_fp64;
-    out->rho_points_fp32 = P.pts_fp32;
+    out->rho_points_fp64 = pts_used[0];
+    out->rho_points_fp32 = pts_used[1];
+    out->flops_rho_fp64_rows = P.flops_rho_fp64_rows;
     out->device_bytes = P.device_bytes();
     out->lds_bytes_per_block = P.lds_limit;
     out->profiled_builds = P.prof.builds;

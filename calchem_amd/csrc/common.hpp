@@ -35,12 +35,26 @@ struct DevAo {
 };
 
 // rho-stage block: a spatially compact set of points and the AOs that are
-// significant at any of them.
+// significant at any of them.  Rows whose |phi| exceeds the mixed-precision
+// threshold somewhere in the block come first (m64 of them, rounded up to 4).
 struct RhoBlock {
-    int ao_begin;  // offset into the block AO list
+    int ao_begin;  // offset into the block row list
     int m;         // significant AOs
-    int m_pad;     // m rounded up to the 32-row tile
-    int pad;
+    int m_pad;     // m rounded up to the 16-row tile
+    int m64;       // leading rows kept in FP64 by the mixed build
+    int sh_begin;  // offset into the block shell list
+    int n_sh;      // shells owning the rows
+    int n_prim;    // primitives of those shells
+    int pad1;
+    long long psub;  // offset of the block's gathered density (m_pad x m_pad, row-major)
+};
+
+// A row as the kernels keep it in LDS: component scale, Cartesian exponents
+// (2 bits each) and the block-local shell (-1 for padding rows).
+struct RowL {
+    double scale;
+    int lmn;
+    int sh;
 };
 
 // One workgroup of the rho kernel: up to kRhoPts consecutive sorted points
@@ -52,22 +66,28 @@ struct RhoTile {
     int pad;
 };
 
-// V-stage group (a union of neighbouring rho blocks).
+// V-stage group (a union of neighbouring rho blocks).  Its rows are the
+// union of the blocks' AOs in ascending order; V tiles of (1 << tile_shift)
+// rows go to scratch as lower tile pairs (I >= J), row-major inside a tile.
 struct VGroup {
-    int ao_begin, m, m_pad, n_tiles64;
+    int ao_begin, m, n_tiles, tile_shift;
     int pt_begin, n_pts;
+    int sh_begin, n_sh;
     long long scratch_off;  // in doubles
+    int n_prim, pad;
 };
 
-// One workgroup of the V kernel: a 64x64 lower tile pair (I >= J) of a group.
+// One workgroup of the tile-pair V kernel: a 64x64 lower tile pair of a
+// group too wide for the register-resident kernel.
 struct VPair {
     int group, ti, tj, pad;
 };
 
-constexpr int kRhoPts = 32;     // points per rho workgroup
-constexpr int kVTile = 64;      // V-stage AO tile
-constexpr int kVPts = 16;       // points staged per V-stage step
-constexpr int kThreads = 256;   // 4 wavefronts of 64
+constexpr int kVTile = 64;       // tile of the tile-pair V kernel
+constexpr int kVPts = 16;        // points staged per V-stage step
+constexpr int kThreads = 256;    // 4 wavefronts of 64
+constexpr int kGroupThreads = 512;   // 8 wavefronts: register-resident V kernel
+constexpr int kGroupMaxTiles = 17;   // 16-row tiles per group it accepts (272 rows)
 
 inline bool hip_ok(hipError_t e, const char* what) {
     if (e != hipSuccess) {

@@ -9,17 +9,31 @@ struct RhoArgs {
     const double2* prims;   // (exponent, c~ N(alpha; L00))
     const DevAo* aos;
     const double4* pts;     // sorted (x, y, z, w)
-    const RhoTile* tiles;
+    const RhoTile* tiles;   // this launch's tiles
     const RhoBlock* blocks;
-    const int* block_ao;    // global AO index per block row
-    const void* P;          // n_ao x n_ao, double or float
-    int n_ao;
-    int func;
-    double4* wv;            // [sorted point] (w v_rho_a / 2, w u grad rho)
-    double4* rho_out;       // [caller point] (rho, grad rho) when func < 0
+    const int2* block_rows; // per block row: (global AO, block-local shell) or (-1, 0)
+    const int2* block_sh;   // per block shell: (global shell, block-local primitive offset)
+    const double* P64;      // gathered FP64 density blocks (RhoBlock::psub)
+    const void* PS;         // gathered density blocks in the precision of the non-leading rows
+    double4* rho_out;       // (rho, grad rho) per sorted point, or per caller point when rho_perm
     const int* perm;        // sorted point -> caller point
-    double* e_part;         // [tile]
-    double* n_part;         // [tile]
+    int rho_perm;
+};
+
+struct XcArgs {
+    const double4* rho;     // per sorted point
+    const double4* pts;     // sorted (x, y, z, w)
+    int n_pts;
+    int func;
+    double4* wv;            // per sorted point
+    double* e_part;         // per workgroup
+    double* n_part;
+};
+
+// Launch class of rho tiles: blocks of similar size share one LDS budget.
+struct RhoClass {
+    int tile_begin, n_tiles;  // range in the tile list of one point granularity
+    int max_mpad, max_nsh, max_m64, max_prim;
 };
 
 struct VArgs {
@@ -28,16 +42,32 @@ struct VArgs {
     const DevAo* aos;
     const double4* pts;     // sorted
     const VGroup* groups;
-    const VPair* pairs;
-    const int* group_ao;
+    const VPair* pairs;     // tile-pair kernel
+    const int* group_list;  // register-resident kernel: groups it handles
+    const int2* group_rows; // per group row: (global AO, group-local shell)
+    const int2* group_sh;   // per group shell: (global shell, group-local primitive offset)
     const double4* wv;      // per sorted point, or per caller point when wv_perm
     const int* perm;
     int wv_perm;
     double* scratch;
 };
 
+// Per build: copy P[row_i][row_j] of every block into its m_pad x m_pad
+// slot (zero for padding rows), in FP64 and/or FP32.
+struct GatherArgs {
+    const RhoBlock* blocks;
+    const int2* block_rows;
+    const double* P;
+    int n_ao;
+    double* out64;          // may be null
+    float* out32;           // may be null
+    int out64_leading_only; // write out64 only for the block's first m64 columns
+};
+
 struct ReduceArgs {
     const int* loc;         // [group][n_ao] group-local row of each AO, or -1
+    const int* ao_gbegin;   // [n_ao + 1] CSR of the groups holding each AO
+    const int* ao_groups;   // group indices, ascending per AO
     const VGroup* groups;
     int n_groups;
     int n_ao;
@@ -53,15 +83,24 @@ struct SigArgs {
     const int* cand_ao;     // [entry] AO index
     const int2* cand_pts;   // [entry] (first sorted point, count) of its block
     unsigned char* flag;    // [entry] 1 if the AO survives the value rule somewhere
+    double* maxabs;         // [entry] max |phi| over the block's points
     long long n_entries;
 };
 
-size_t rho_lds_bytes(int prec_bytes, int pts, int max_mpad);
-hipError_t launch_rho(const RhoArgs& a, bool fp32, int pts, bool xc, int n_tiles, int max_mpad, hipStream_t s);
+// Rho kernel variants: FP64 throughout, or mixed (leading rows FP64, the
+// rest FP32).  pts = 32 or 16 points per workgroup.
+enum RhoMode { kRhoFp64 = 0, kRhoMixed = 1 };
+size_t rho_lds_bytes(int mode, int pts, const RhoClass& c);
+hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hipStream_t s);
+hipError_t launch_xc(const XcArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);
+size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim);
+hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh, int max_prim, hipStream_t s);
 hipError_t launch_reduce_v(const ReduceArgs& a, hipStream_t s);
 hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s);
 hipError_t launch_to_float(const double* x, float* y, long long n, hipStream_t s);
+hipError_t launch_fill_zero(double* x, long long n, hipStream_t s);
+hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, hipStream_t s);
 hipError_t launch_significance(const SigArgs& a, hipStream_t s);
 
 // Grid construction (grid.hip).

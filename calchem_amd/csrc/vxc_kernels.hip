@@ -1,14 +1,17 @@
 // vxc_kernels.hip — gfx950 kernels of the RKS Vxc / E_xc / N build.
 //
 // One build (crates/calchem-dft/src/vxc.rs:801-985 restated for CDNA4):
-//   k_rho     per (block, 32 or 16 points): AO values of the block's AO list
-//             evaluated straight into LDS, T = P_sub Phi on MFMA (P rows read
-//             from L2), rho = sum_i Phi_i T_i and grad rho = 2 sum_i dPhi_i T_i,
-//             then the functional in FP64: per-point (w v_rho/2, w u grad rho)
+//   k_rho     per (block, 32 or 16 points): the block's shells, primitives and
+//             AO rows staged in LDS, radial sums per (shell, point), AO values
+//             into LDS, T = P_sub Phi on MFMA (P_sub rows read from L2),
+//             rho = sum_i Phi_i T_i and grad rho = 2 sum_i dPhi_i T_i.
+//   k_xc      per point: the functional in FP64, (w v_rho/2, w u grad rho)
 //             and per-workgroup partial E_xc and N.
-//   k_vxc     per (group, 64x64 lower tile pair): Z = w v_rho/2 Phi + wu . dPhi
-//             and Phi staged in LDS 16 points at a time, acc += Z_I Phi_J^T +
-//             Phi_I Z_J^T on MFMA; the tile goes to a per-group scratch slot.
+//   k_vxc_group  per group (up to 272 AO rows): every lower 16x16 tile pair
+//             of the group's V in registers for the whole pass over its
+//             points; Z = w v_rho/2 Phi + wu . dPhi and Phi staged in LDS 16
+//             points at a time, acc += Z_I Phi_J^T + Phi_I Z_J^T on MFMA.
+//   k_vxc     the same per 64x64 tile pair, for groups too wide for the above.
 //   k_reduce_v   deterministic sum of the group tiles into the full symmetric V.
 //   k_reduce_en  deterministic sum of the E_xc / N partials.
 // T = double runs FP64 MFMA (v_mfma_f64_16x16x4f64); T = float runs FP32 MFMA
@@ -46,6 +49,17 @@ struct Mma<float> {
     }
 };
 
+template <typename T>
+struct Vec4;
+template <>
+struct Vec4<double> {
+    using type = double4;
+};
+template <>
+struct Vec4<float> {
+    using type = float4;
+};
+
 // MFMA 16x16x4 operand layout (one value per lane): A[i][k] at lane i + 16 k,
 // B[k][j] at lane j + 16 k.  Element v of lane j + 16 q of the 16x16 result
 // holds D[Mma<T>::row(q, v)][j]: row 4 q + v for the f32 instruction, row
@@ -54,78 +68,153 @@ struct Mma<float> {
 // ---------------------------------------------------------------------------
 // rho stage
 // ---------------------------------------------------------------------------
+// Row stride (elements) of the LDS value tables: rows of one MFMA k-step land
+// on disjoint LDS banks for both halves of a wavefront.
 template <int PTS>
-__host__ __device__ constexpr size_t rho_lds_fixed() {
-    return sizeof(double4) * PTS + sizeof(double) * 16 * PTS + sizeof(double) * 2 * PTS;
+__host__ __device__ constexpr int rho_ldp() {
+    return PTS == 32 ? 48 : 16;
 }
 
-template <typename T, int PTS, bool XC>
+// Dynamic LDS of one rho workgroup, carved in this order (every piece is a
+// multiple of 16 bytes): points, per-wave sums, radial table [shell][point],
+// shells, primitives, rows, FP64 leading-row values, row values.
+template <int PTS>
+__host__ __device__ constexpr size_t rho_lds_fixed() {
+    return sizeof(double4) * PTS + sizeof(double) * 16 * PTS;
+}
+
+// Position in an f64 16x16x4 A tile that makes its result element v of lane
+// (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
+__device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >> 2); }
+
+template <typename TS, int PTS, bool HYB>
 __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int LDP = rho_ldp<PTS>();
     const RhoTile tile = a.tiles[blockIdx.x];
     const RhoBlock blk = a.blocks[tile.block];
-    const int m = blk.m, mp = blk.m_pad, np = tile.n_pts;
-    double4* sPt = reinterpret_cast<double4*>(smem);
-    double* sRed = reinterpret_cast<double*>(sPt + PTS);  // [4 waves][4 sums][PTS]
-    double* sEN = sRed + 16 * PTS;                         // [2][PTS]
-    int* sAo = reinterpret_cast<int*>(sEN + 2 * PTS);      // [mp]
-    T* sPhi = reinterpret_cast<T*>(sAo + mp);              // [mp][PTS]
+    const int m = blk.m, mp = blk.m_pad, np = tile.n_pts, nsh = blk.n_sh;
+    const int m64 = HYB ? blk.m64 : 0;
+    unsigned char* q = smem;
+    double4* sPt = reinterpret_cast<double4*>(q);
+    q += sizeof(double4) * PTS;
+    double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 sums][PTS]
+    q += sizeof(double) * 16 * PTS;
+    double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
+    q += sizeof(double2) * nsh * PTS;
+    DevShell* sSh = reinterpret_cast<DevShell*>(q);
+    q += sizeof(DevShell) * nsh;
+    double2* sPrim = reinterpret_cast<double2*>(q);
+    q += sizeof(double2) * blk.n_prim;
+    RowL* sRow = reinterpret_cast<RowL*>(q);
+    q += sizeof(RowL) * mp;
+    double* sPhi64 = reinterpret_cast<double*>(q);  // [m64][LDP]
+    q += HYB ? sizeof(double) * m64 * LDP : 0;
+    TS* sPhi = reinterpret_cast<TS*>(q);  // [mp][LDP]
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    for (int r = tid; r < mp; r += kThreads) sAo[r] = r < m ? a.block_ao[blk.ao_begin + r] : -1;
+    stage_meta(a.block_rows + blk.ao_begin, m, mp, a.block_sh + blk.sh_begin, nsh, a.shells, a.prims, a.aos, sRow,
+               sSh, sPrim, tid, kThreads);
     if (tid < PTS) sPt[tid] = tid < np ? a.pts[tile.pt_begin + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
 
-    // AO values of every listed AO at the tile's points, straight into LDS.
-    for (int idx = tid; idx < mp * PTS; idx += kThreads) {
-        const int r = idx / PTS, p = idx % PTS;
-        const int mu = sAo[r];
-        double v = 0.0;
-        if (mu >= 0 && p < np) {
-            const double4 q = sPt[p];
-            v = ao_eval(a.shells, a.prims, a.aos[mu], q.x, q.y, q.z).v;
+    // Radial sums once per (shell, point).
+    for (int idx = tid; idx < nsh * PTS; idx += kThreads) {
+        const int sh = idx / PTS, p = idx % PTS;
+        double2 rr = make_double2(0.0, 0.0);
+        if (p < np) {
+            const double4 x = sPt[p];
+            rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
         }
-        sPhi[idx] = static_cast<T>(v);
+        sRad[idx] = rr;
     }
     __syncthreads();
 
-    using M = Mma<T>;
+    // AO values of every row at every point of the tile.
+    for (int idx = tid; idx < mp * PTS; idx += kThreads) {
+        const int r = idx / PTS, p = idx % PTS;
+        const RowL row = sRow[r];
+        double v = 0.0;
+        if (row.sh >= 0) {
+            const double4 x = sPt[p];
+            v = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + p]).v;
+        }
+        sPhi[r * LDP + p] = static_cast<TS>(v);
+        if (HYB && r < m64) sPhi64[r * LDP + p] = v;
+    }
+    __syncthreads();
+
+    using M = Mma<TS>;
     constexpr int NH = PTS / 16;
-    const T* __restrict__ P = static_cast<const T*>(a.P);
-    const int n = a.n_ao;
     const int col = lane & 15, kq = lane >> 4;
+    const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
     double sr[NH], sx[NH], sy[NH], sz[NH];
 #pragma unroll
     for (int h = 0; h < NH; ++h) sr[h] = sx[h] = sy[h] = sz[h] = 0.0;
 
     for (int it = wave; it < (mp >> 4); it += 4) {
         const int i0 = it << 4;
-        const int arow = sAo[i0 + col];
-        const T* __restrict__ Prow = P + static_cast<size_t>(arow < 0 ? 0 : arow) * n;
         typename M::acc acc[NH];
 #pragma unroll
         for (int h = 0; h < NH; ++h) acc[h] = M::zero();
-#pragma unroll 4
-        for (int k = 0; k < mp; k += 4) {
-            const int kk = k + kq;
-            const int acol = sAo[kk];
-            const T av = (arow >= 0 && acol >= 0) ? Prow[acol] : T(0);
+        v4d acc64[NH];
+        if constexpr (HYB) {
 #pragma unroll
-            for (int h = 0; h < NH; ++h) acc[h] = M::op(av, sPhi[kk * PTS + h * 16 + col], acc[h]);
+            for (int h = 0; h < NH; ++h) acc64[h] = Mma<double>::zero();
+            // Leading rows (large |phi|) in FP64 with the FP64 density block.
+            const double* __restrict__ Arow = a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
+            for (int k = 0; k < m64; k += 4) {
+                const int kk = k + kq;
+                const double av = Arow[kk];
+#pragma unroll
+                for (int h = 0; h < NH; ++h)
+                    acc64[h] = Mma<double>::op(av, sPhi64[kk * LDP + h * 16 + col], acc64[h]);
+            }
+        }
+        {
+            // K is visited in chunks of 16 whose slots are permuted so that
+            // lane (i, q) reads four consecutive density entries per load:
+            // step s of a chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].
+            const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>(i0 + col) * mp;
+            int k = m64;
+            for (; k < mp && (k & 15); k += 4) {
+                const int kk = k + kq;
+                const TS av = Arow[kk];
+#pragma unroll
+                for (int h = 0; h < NH; ++h) acc[h] = M::op(av, sPhi[kk * LDP + h * 16 + col], acc[h]);
+            }
+            using V4 = typename Vec4<TS>::type;
+            if (k < mp) {
+                V4 cur = *reinterpret_cast<const V4*>(Arow + k + 4 * kq);
+                for (; k < mp; k += 16) {
+                    V4 nxt = cur;
+                    if (k + 16 < mp) nxt = *reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq);
+                    const int kb = k + 4 * kq;
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) {
+                        acc[h] = M::op(cur.x, sPhi[(kb + 0) * LDP + h * 16 + col], acc[h]);
+                        acc[h] = M::op(cur.y, sPhi[(kb + 1) * LDP + h * 16 + col], acc[h]);
+                        acc[h] = M::op(cur.z, sPhi[(kb + 2) * LDP + h * 16 + col], acc[h]);
+                        acc[h] = M::op(cur.w, sPhi[(kb + 3) * LDP + h * 16 + col], acc[h]);
+                    }
+                    cur = nxt;
+                }
+            }
         }
         // Epilogue: this lane holds T[i0 + row(kq, v)][h*16 + col].
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
-            const int mu = sAo[i0 + M::row(kq, v)];
-            if (mu < 0) continue;
-            const DevAo ao = a.aos[mu];
+            const RowL row = sRow[i0 + M::row(kq, v)];
+            if (row.sh < 0) continue;
+            const DevShell sh = sSh[row.sh];
 #pragma unroll
             for (int h = 0; h < NH; ++h) {
                 const int p = h * 16 + col;
                 if (p >= np) continue;
-                const double t = static_cast<double>(acc[h][v]);
-                const double4 q = sPt[p];
-                const AoVal f = ao_eval(a.shells, a.prims, ao, q.x, q.y, q.z);
+                double t = static_cast<double>(acc[h][v]);
+                if constexpr (HYB) t += acc64[h][v];
+                const double4 x = sPt[p];
+                const AoVal f = ao_from_radial_row(sh, row, x.x, x.y, x.z, sRad[row.sh * PTS + p]);
                 sr[h] += f.v * t;
                 sx[h] += f.gx * t;
                 sy[h] += f.gy * t;
@@ -153,7 +242,7 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
         }
     }
     __syncthreads();
-    if (tid < PTS) {
+    if (tid < np) {
         const int p = tid;
         double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
@@ -163,43 +252,46 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
             gy += sRed[(w * 4 + 2) * PTS + p];
             gz += sRed[(w * 4 + 3) * PTS + p];
         }
-        gx *= 2.0;
-        gy *= 2.0;
-        gz *= 2.0;
-        double e = 0.0, ne = 0.0;
-        if (p < np) {
-            const int g = tile.pt_begin + p;
-            if constexpr (XC) {
-                const double w = sPt[p].w;
-                ne = w * rho;
-                double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
-                if (rho >= 1e-18) {  // DENSITY_SCREEN, vxc.rs:23
-                    const double rs = 0.5 * rho;
-                    const double gss = 0.25 * (gx * gx + gy * gy + gz * gz);
-                    const XcOut o = eval_xc(a.func, rs, rs, gss, gss, gss);
-                    e = w * o.f;
-                    const double us = w * (o.gaa + 0.5 * o.gab);
-                    out = make_double4(0.5 * w * o.va, us * gx, us * gy, us * gz);
-                }
-                a.wv[g] = out;
-            } else {
-                a.rho_out[a.perm[g]] = make_double4(rho, gx, gy, gz);
-            }
-        }
-        sEN[p] = e;
-        sEN[PTS + p] = ne;
+        const int g = tile.pt_begin + p;
+        a.rho_out[a.rho_perm ? a.perm[g] : g] = make_double4(rho, 2.0 * gx, 2.0 * gy, 2.0 * gz);
     }
-    if constexpr (XC) {
-        __syncthreads();
-        if (tid == 0) {
-            double e = 0.0, ne = 0.0;
-            for (int p = 0; p < PTS; ++p) {
-                e += sEN[p];
-                ne += sEN[PTS + p];
-            }
-            a.e_part[blockIdx.x] = e;
-            a.n_part[blockIdx.x] = ne;
+}
+
+// Functional stage, one thread per sorted point: from (rho, grad rho) and the
+// weight, wv = (w v_rho_a / 2, w (v_gamma_aa + v_gamma_ab / 2) grad rho) and
+// the workgroup's sums of w f_xc and w rho (vxc.rs:892-920).
+__global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
+    __shared__ double se[kThreads], sn[kThreads];
+    const int g = blockIdx.x * kThreads + threadIdx.x;
+    double e = 0.0, ne = 0.0;
+    if (g < a.n_pts) {
+        const double4 r = a.rho[g];
+        const double w = a.pts[g].w;
+        ne = w * r.x;
+        double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (r.x >= 1e-18) {  // DENSITY_SCREEN, vxc.rs:23
+            const double rs = 0.5 * r.x;
+            const double gss = 0.25 * (r.y * r.y + r.z * r.z + r.w * r.w);
+            const XcOut o = eval_xc(a.func, rs, rs, gss, gss, gss);
+            e = w * o.f;
+            const double us = w * (o.gaa + 0.5 * o.gab);
+            out = make_double4(0.5 * w * o.va, us * r.y, us * r.z, us * r.w);
         }
+        a.wv[g] = out;
+    }
+    se[threadIdx.x] = e;
+    sn[threadIdx.x] = ne;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sn[threadIdx.x] += sn[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.e_part[blockIdx.x] = se[0];
+        a.n_part[blockIdx.x] = sn[0];
     }
 }
 
@@ -222,8 +314,8 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
 
     if (tid < kVTile) {
         const int gi = pr.ti * kVTile + tid, gj = pr.tj * kVTile + tid;
-        sAoI[tid] = gi < G.m ? a.group_ao[G.ao_begin + gi] : -1;
-        sAoJ[tid] = gj < G.m ? a.group_ao[G.ao_begin + gj] : -1;
+        sAoI[tid] = gi < G.m ? a.group_rows[G.ao_begin + gi].x : -1;
+        sAoJ[tid] = gj < G.m ? a.group_rows[G.ao_begin + gj].x : -1;
     }
     using M = Mma<T>;
     typename M::acc acc[2][2];
@@ -301,38 +393,183 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
             }
 }
 
+// Register-resident V kernel: one 512-thread workgroup per group.  Every
+// lower pair of 16-row tiles of the group's V lives in the accumulators of
+// one wavefront for the whole pass over the group's points; 16 points at a
+// time, radial sums (per shell) then phi and Z = w v_rho/2 phi + wu . dphi
+// (per row) are staged in LDS, each evaluated once, and every pair takes
+// acc += Z_I Phi_J^T + Phi_I Z_J^T.
+constexpr int kPairsPerWave = (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + 7) / 8;
+
+template <typename T>
+__host__ __device__ constexpr size_t vgroup_lds_fixed() {
+    return sizeof(double4) * 2 * kVPts + sizeof(RowL) * kGroupMaxTiles * 16 +
+           sizeof(T) * 2 * kGroupMaxTiles * 16 * (kVPts + 1);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int LD = kVPts + 1;
+    const VGroup G = a.groups[a.group_list[blockIdx.x]];
+    const int m = G.m, nsh = G.n_sh;
+    const int T16 = (m + 15) >> 4, mp = T16 << 4;
+    const int npairs = T16 * (T16 + 1) / 2;
+    unsigned char* q = smem;
+    double4* sPt = reinterpret_cast<double4*>(q);
+    q += sizeof(double4) * kVPts;
+    double4* sWv = reinterpret_cast<double4*>(q);
+    q += sizeof(double4) * kVPts;
+    RowL* sRow = reinterpret_cast<RowL*>(q);
+    q += sizeof(RowL) * kGroupMaxTiles * 16;
+    T* sP = reinterpret_cast<T*>(q);
+    q += sizeof(T) * kGroupMaxTiles * 16 * LD;
+    T* sZ = reinterpret_cast<T*>(q);
+    q += sizeof(T) * kGroupMaxTiles * 16 * LD;
+    double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][kVPts]
+    q += sizeof(double2) * nsh * kVPts;
+    DevShell* sSh = reinterpret_cast<DevShell*>(q);
+    q += sizeof(DevShell) * nsh;
+    double2* sPrim = reinterpret_cast<double2*>(q);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+    stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
+               sPrim, tid, kGroupThreads);
+
+    using M = Mma<T>;
+    typename M::acc acc[kPairsPerWave];
+    int ptij[kPairsPerWave];  // ti << 16 | tj
+#pragma unroll
+    for (int w = 0; w < kPairsPerWave; ++w) {
+        acc[w] = M::zero();
+        // pair index p = ti (ti + 1) / 2 + tj, handed out round robin to the waves
+        const int p = wave + 8 * w;
+        int ti = 0;
+        while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
+        ptij[w] = (ti << 16) | (p - ti * (ti + 1) / 2);
+    }
+
+    for (int q0 = 0; q0 < G.n_pts; q0 += kVPts) {
+        __syncthreads();
+        if (tid < kVPts) {
+            const int g = q0 + tid;
+            if (g < G.n_pts) {
+                const int sg = G.pt_begin + g;
+                sPt[tid] = a.pts[sg];
+                sWv[tid] = a.wv[a.wv_perm ? a.perm[sg] : sg];
+            } else {
+                sPt[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+                sWv[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+            }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < nsh * kVPts; idx += kGroupThreads) {
+            const int sh = idx / kVPts, p = idx % kVPts;
+            double2 rr = make_double2(0.0, 0.0);
+            if (q0 + p < G.n_pts) {
+                const double4 x = sPt[p];
+                rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
+            }
+            sRad[idx] = rr;
+        }
+        __syncthreads();
+        for (int idx = tid; idx < mp * kVPts; idx += kGroupThreads) {
+            const int r = idx / kVPts, p = idx % kVPts;
+            const RowL row = sRow[r];
+            double v = 0.0, z = 0.0;
+            if (row.sh >= 0) {
+                const double4 x = sPt[p];
+                const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
+                const double4 w = sWv[p];
+                v = f.v;
+                z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
+            }
+            sP[r * LD + p] = static_cast<T>(v);
+            sZ[r * LD + p] = static_cast<T>(z);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kPairsPerWave; ++w) {
+            if (wave + 8 * w >= npairs) continue;
+            const int ri = ((ptij[w] >> 16) * 16 + col) * LD, rj = ((ptij[w] & 0xffff) * 16 + col) * LD;
+#pragma unroll
+            for (int k = 0; k < kVPts; k += 4) {
+                const int kk = k + kq;
+                acc[w] = M::op(sZ[ri + kk], sP[rj + kk], acc[w]);
+                acc[w] = M::op(sP[ri + kk], sZ[rj + kk], acc[w]);
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < kPairsPerWave; ++w) {
+        const int p = wave + 8 * w;
+        if (p >= npairs) continue;
+        double* out = a.scratch + G.scratch_off + static_cast<long long>(p) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[M::row(kq, v) * 16 + col] = static_cast<double>(acc[w][v]);
+    }
+}
+
+// V[mu][nu] (mu >= nu) = sum over the groups holding mu, in ascending group
+// order, of that group's tile element; groups without nu add nothing.  Four
+// groups are looked up per round so their loads overlap.
+__device__ __forceinline__ double group_elem(const ReduceArgs& a, int g, int mu, int nu) {
+    const int n = a.n_ao;
+    const int lj = a.loc[static_cast<long long>(g) * n + nu];
+    if (lj < 0) return 0.0;
+    const int li = a.loc[static_cast<long long>(g) * n + mu];
+    const VGroup G = a.groups[g];
+    const int ts = G.tile_shift, mask = (1 << ts) - 1;
+    const int ti = li >> ts, tj = lj >> ts;
+    return a.scratch[G.scratch_off + (static_cast<long long>(ti * (ti + 1) / 2 + tj) << (2 * ts)) +
+                     ((li & mask) << ts) + (lj & mask)];
+}
+
 __global__ __launch_bounds__(kThreads) void k_reduce_v(ReduceArgs a) {
     const int mu = blockIdx.y;
     const int nu = blockIdx.x * kThreads + threadIdx.x;
     if (nu > mu || nu >= a.n_ao) return;
     const int n = a.n_ao;
+    const int e0 = a.ao_gbegin[mu], e1 = a.ao_gbegin[mu + 1];
     double s = 0.0;
-    for (int g = 0; g < a.n_groups; ++g) {
-        const int li = a.loc[static_cast<long long>(g) * n + mu];
-        if (li < 0) continue;
-        const int lj = a.loc[static_cast<long long>(g) * n + nu];
-        if (lj < 0) continue;
-        const int ti = li >> 6, tj = lj >> 6;
-        s += a.scratch[a.groups[g].scratch_off +
-                       static_cast<long long>(ti * (ti + 1) / 2 + tj) * (kVTile * kVTile) + (li & 63) * kVTile +
-                       (lj & 63)];
+    int e = e0;
+    for (; e + 4 <= e1; e += 4) {
+        const double v0 = group_elem(a, a.ao_groups[e], mu, nu);
+        const double v1 = group_elem(a, a.ao_groups[e + 1], mu, nu);
+        const double v2 = group_elem(a, a.ao_groups[e + 2], mu, nu);
+        const double v3 = group_elem(a, a.ao_groups[e + 3], mu, nu);
+        s += v0;
+        s += v1;
+        s += v2;
+        s += v3;
     }
+    for (; e < e1; ++e) s += group_elem(a, a.ao_groups[e], mu, nu);
     a.V[static_cast<long long>(mu) * n + nu] = s;
     a.V[static_cast<long long>(nu) * n + mu] = s;
 }
 
-__global__ __launch_bounds__(kThreads) void k_reduce_en(const double* __restrict__ e, const double* __restrict__ ne,
-                                                        int n, double* __restrict__ out) {
-    __shared__ double se[kThreads], sn[kThreads];
-    double x = 0.0, y = 0.0;
-    for (int i = threadIdx.x; i < n; i += kThreads) {
-        x += e[i];
-        y += ne[i];
+__global__ __launch_bounds__(1024) void k_reduce_en(const double* __restrict__ e, const double* __restrict__ ne,
+                                                    int n, double* __restrict__ out) {
+    __shared__ double se[1024], sn[1024];
+    double x[4] = {0.0, 0.0, 0.0, 0.0}, y[4] = {0.0, 0.0, 0.0, 0.0};
+    int i = threadIdx.x;
+    for (; i + 3 * 1024 < n; i += 4 * 1024) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] += e[i + u * 1024];
+            y[u] += ne[i + u * 1024];
+        }
     }
-    se[threadIdx.x] = x;
-    sn[threadIdx.x] = y;
+    for (; i < n; i += 1024) {
+        x[0] += e[i];
+        y[0] += ne[i];
+    }
+    se[threadIdx.x] = (x[0] + x[1]) + (x[2] + x[3]);
+    sn[threadIdx.x] = (y[0] + y[1]) + (y[2] + y[3]);
     __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
+    for (int s = 512; s > 0; s >>= 1) {
         if (threadIdx.x < s) {
             se[threadIdx.x] += se[threadIdx.x + s];
             sn[threadIdx.x] += sn[threadIdx.x + s];
@@ -350,8 +587,25 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
     if (i < n) y[i] = static_cast<float>(x[i]);
 }
 
-// Per (group-local candidate AO): does the AO survive the reference's value
-// rule at any point of its block?  Setup only.
+__global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
+    const RhoBlock B = a.blocks[blockIdx.x];
+    const int mp = B.m_pad;
+    for (int idx = threadIdx.x; idx < mp * mp; idx += kThreads) {
+        const int i = idx / mp, j = idx % mp;
+        const int ri = a.block_rows[B.ao_begin + i].x, rj = a.block_rows[B.ao_begin + j].x;
+        const double v = (ri >= 0 && rj >= 0) ? a.P[static_cast<long long>(ri) * a.n_ao + rj] : 0.0;
+        if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[B.psub + idx] = v;
+        if (a.out32) a.out32[B.psub + idx] = static_cast<float>(v);
+    }
+}
+
+__global__ void k_fill_zero(double* __restrict__ x, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+// Per (block, candidate AO): does the AO survive the reference's value rule
+// at any point of the block, and its largest |phi| there.  Setup only.
 __global__ void k_significance(SigArgs a) {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= a.n_entries) return;
@@ -359,48 +613,60 @@ __global__ void k_significance(SigArgs a) {
     const int2 rng = a.cand_pts[i];  // sorted point range of the entry's block
     const DevAo ao = a.aos[mu];
     unsigned char keep = 0;
-    for (int g = rng.x; g < rng.x + rng.y && !keep; ++g) {
+    double mx = 0.0;
+    for (int g = rng.x; g < rng.x + rng.y; ++g) {
         const double4 q = a.pts[g];
         const AoVal f = ao_eval(a.shells, a.prims, ao, q.x, q.y, q.z);
-        keep = (f.v != 0.0 || f.gx != 0.0 || f.gy != 0.0 || f.gz != 0.0);
+        keep |= (f.v != 0.0 || f.gx != 0.0 || f.gy != 0.0 || f.gz != 0.0);
+        mx = fmax(mx, fabs(f.v));
     }
     a.flag[i] = keep;
+    a.maxabs[i] = mx;
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-template <typename T, int PTS, bool XC>
-static hipError_t launch_rho_t(const RhoArgs& a, int n_tiles, int max_mpad, hipStream_t s) {
-    const size_t lds = rho_lds_fixed<PTS>() + sizeof(int) * max_mpad + sizeof(T) * (size_t)max_mpad * PTS;
-    static bool attr_set = false;
+template <typename TS, int PTS, bool HYB>
+static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
+    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, PTS, c);
+    static int attr_set = 0;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<T, PTS, XC>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        int dev = 0, lim = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, PTS, HYB>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
-        attr_set = true;
+        attr_set = 1;
     }
-    if (n_tiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL((k_rho<T, PTS, XC>), dim3(n_tiles), dim3(kThreads), lds, s, a);
+    if (c.n_tiles <= 0) return hipSuccess;
+    RhoArgs b = a;
+    b.tiles = a.tiles + c.tile_begin;
+    hipLaunchKernelGGL((k_rho<TS, PTS, HYB>), dim3(c.n_tiles), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
-size_t rho_lds_bytes(int prec_bytes, int pts, int max_mpad) {
+size_t rho_lds_bytes(int mode, int pts, const RhoClass& c) {
     const size_t fixed = pts == 32 ? rho_lds_fixed<32>() : rho_lds_fixed<16>();
-    return fixed + sizeof(int) * (size_t)max_mpad + (size_t)prec_bytes * max_mpad * pts;
+    const size_t ldp = pts == 32 ? rho_ldp<32>() : rho_ldp<16>();
+    const size_t ts = mode == kRhoMixed ? sizeof(float) : sizeof(double);
+    return fixed + sizeof(double2) * (size_t)c.max_nsh * pts + sizeof(DevShell) * (size_t)c.max_nsh +
+           sizeof(double2) * (size_t)c.max_prim + sizeof(RowL) * (size_t)c.max_mpad +
+           (mode == kRhoMixed ? sizeof(double) * (size_t)c.max_m64 * ldp : 0) + ts * (size_t)c.max_mpad * ldp;
 }
 
-hipError_t launch_rho(const RhoArgs& a, bool fp32, int pts, bool xc, int n_tiles, int max_mpad, hipStream_t s) {
-    if (!fp32) {
-        if (pts == 32) return xc ? launch_rho_t<double, 32, true>(a, n_tiles, max_mpad, s)
-                                 : launch_rho_t<double, 32, false>(a, n_tiles, max_mpad, s);
-        return xc ? launch_rho_t<double, 16, true>(a, n_tiles, max_mpad, s)
-                  : launch_rho_t<double, 16, false>(a, n_tiles, max_mpad, s);
-    }
-    if (pts == 32) return xc ? launch_rho_t<float, 32, true>(a, n_tiles, max_mpad, s)
-                             : launch_rho_t<float, 32, false>(a, n_tiles, max_mpad, s);
-    return xc ? launch_rho_t<float, 16, true>(a, n_tiles, max_mpad, s)
-              : launch_rho_t<float, 16, false>(a, n_tiles, max_mpad, s);
+hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hipStream_t s) {
+    if (mode == kRhoFp64)
+        return pts == 32 ? launch_rho_t<double, 32, false>(a, c, s) : launch_rho_t<double, 16, false>(a, c, s);
+    return pts == 32 ? launch_rho_t<float, 32, true>(a, c, s) : launch_rho_t<float, 16, true>(a, c, s);
+}
+
+hipError_t launch_xc(const XcArgs& a, hipStream_t s) {
+    if (a.n_pts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xc, dim3((a.n_pts + kThreads - 1) / kThreads), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
@@ -412,6 +678,35 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
     return hipGetLastError();
 }
 
+size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim) {
+    const size_t fixed = fp32 ? vgroup_lds_fixed<float>() : vgroup_lds_fixed<double>();
+    return fixed + (sizeof(double2) * kVPts + sizeof(DevShell)) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
+}
+
+template <typename T>
+static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, int max_prim, hipStream_t s) {
+    static int attr_set = 0;
+    if (!attr_set) {
+        int dev = 0, lim = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vxc_group<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+        if (e != hipSuccess) return e;
+        attr_set = 1;
+    }
+    if (n_groups <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_vxc_group<T>, dim3(n_groups), dim3(kGroupThreads),
+                       vgroup_lds_bytes(sizeof(T) == 4, max_nsh, max_prim), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh, int max_prim, hipStream_t s) {
+    return fp32 ? launch_vxc_group_t<float>(a, n_groups, max_nsh, max_prim, s)
+                : launch_vxc_group_t<double>(a, n_groups, max_nsh, max_prim, s);
+}
+
 hipError_t launch_reduce_v(const ReduceArgs& a, hipStream_t s) {
     if (a.n_ao <= 0) return hipSuccess;
     dim3 grid((a.n_ao + kThreads - 1) / kThreads, a.n_ao);
@@ -420,13 +715,25 @@ hipError_t launch_reduce_v(const ReduceArgs& a, hipStream_t s) {
 }
 
 hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_en, dim3(1), dim3(kThreads), 0, s, e, ne, n, out);
+    hipLaunchKernelGGL(k_reduce_en, dim3(1), dim3(1024), 0, s, e, ne, n, out);
     return hipGetLastError();
 }
 
 hipError_t launch_to_float(const double* x, float* y, long long n, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(double* x, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, hipStream_t s) {
+    if (n_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_p, dim3(n_blocks), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -74,9 +74,10 @@ enum {
 };
 
 typedef struct {
-    int32_t block_points;       /* max points per rho-stage block (default 256) */
-    int32_t group_points;       /* target points per V-stage group (default 1024) */
-    size_t memory_limit_bytes;  /* 0 = no limit */
+    int32_t block_points;          /* max points per rho-stage block (default 256) */
+    int32_t group_points;          /* target points per V-stage group (default 1024) */
+    size_t memory_limit_bytes;     /* 0 = no limit */
+    double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.25) */
     int32_t reserved[4];
 } CalchemVxcOptions;
 
@@ -95,6 +96,7 @@ typedef struct {
     double ms_vxc;
     double ms_reduce_v;
     double ms_reduce_en;
+    double flops_rho_fp64_rows;      /* part of flops_rho over the rows a mixed build keeps in FP64 */
 } CalchemVxcStats;
 
 /* ---- Device queries: crates/calchem-cuda/src/lib.rs:469-489 (is_cuda_available,
