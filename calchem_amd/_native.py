@@ -41,8 +41,8 @@ class VxcStats(C.Structure):
                 ("max_m_rho", C.c_int32), ("max_m_v", C.c_int32), ("rho_points_fp64", C.c_int32),
                 ("rho_points_fp32", C.c_int32), ("device_bytes", C.c_size_t),
                 ("lds_bytes_per_block", C.c_size_t),
-                ("profiled_builds", C.c_int64), ("ms_convert_and_rho", C.c_double),
-                ("ms_vxc", C.c_double), ("ms_reduce_v", C.c_double), ("ms_reduce_en", C.c_double),
+                ("profiled_builds", C.c_int64), ("ms_gather", C.c_double), ("ms_rho", C.c_double),
+                ("ms_xc", C.c_double), ("ms_vxc", C.c_double), ("ms_reduce_v", C.c_double), ("ms_reduce_en", C.c_double),
                 ("flops_rho_fp64_rows", C.c_double)]
 
     def as_dict(self) -> dict:

@@ -55,8 +55,9 @@ struct DevBuf {
 
 struct Profile {
     bool on = false;
-    std::vector<hipEvent_t> ev;  // 5 events per build: start, after rho, after vxc, after reduce, end
-    double ms[4] = {0, 0, 0, 0};
+    static constexpr int kMarks = 7;  // start, after gather, rho, xc, vxc, reduce_v, reduce_en
+    std::vector<hipEvent_t> ev;
+    double ms[kMarks - 1] = {0, 0, 0, 0, 0, 0};
     long builds = 0;
 };
 
@@ -76,9 +77,9 @@ struct Plan {
     double flops_rho = 0, flops_v = 0, flops_rho_pad[2] = {0, 0}, flops_v_pad = 0, flops_rho_fp64_rows = 0;
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
-    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, n_xc_blocks = 0;
+    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, n_xc_blocks = 0, reduce_waves = 1;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, tiles32, tiles16, groups, group_rows,
-        group_sh, group_list, pairs, loc, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, P32,
+        group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, P32,
         V, rho, rho_s, psub64, psub32;
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
@@ -91,7 +92,7 @@ struct Plan {
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
                              &block_sh, &tiles32,   &tiles16,  &groups, &group_rows, &group_sh, &group_list,
-                             &pairs,    &loc,       &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
+                             &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &P32,       &V,        &rho, &rho_s, &psub64, &psub32};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
@@ -381,7 +382,6 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int2> hgrows, hgsh;  // hgsh: (global shell, group-local primitive offset)
     std::vector<int> hglist;
     std::vector<VPair> hpairs;
-    std::vector<int> hloc;
     long long scratch = 0;
     const int max_rows = kGroupMaxTiles * 16;
     auto union_rows = [&](int b, int e) {
@@ -438,9 +438,6 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                                  (double)round_up(npt, kVPts);
             }
             scratch += (long long)G.n_tiles * (G.n_tiles + 1) / 2 << (2 * G.tile_shift);
-            const size_t lo = hloc.size();
-            hloc.resize(lo + n_ao, -1);
-            for (int i = 0; i < m; ++i) hloc[lo + u[i]] = i;
             P.flops_v += 2.0 * m * (double)m * npt;
             P.max_m_v = std::max(P.max_m_v, m);
             hg.push_back(G);
@@ -450,8 +447,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     P.n_groups = static_cast<int>(hg.size());
     P.n_pairs = static_cast<int>(hpairs.size());
     P.n_group_list = static_cast<int>(hglist.size());
-    // CSR: for each AO the groups holding it, ascending.
-    std::vector<int> hgb(n_ao + 1, 0), hgg;
+    // CSR: for each AO the groups holding it, ascending, with its row there.
+    std::vector<int> hgb(n_ao + 1, 0);
+    std::vector<RedEntry> hgg;
     for (int g = 0; g < P.n_groups; ++g)
         for (int i = 0; i < hg[g].m; ++i) hgb[hgrows[hg[g].ao_begin + i].x + 1]++;
     for (int mu = 0; mu < n_ao; ++mu) hgb[mu + 1] += hgb[mu];
@@ -459,7 +457,15 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     {
         std::vector<int> fill(hgb.begin(), hgb.end() - 1);
         for (int g = 0; g < P.n_groups; ++g)
-            for (int i = 0; i < hg[g].m; ++i) hgg[fill[hgrows[hg[g].ao_begin + i].x]++] = g;
+            for (int i = 0; i < hg[g].m; ++i) {
+                const VGroup& G = hg[g];
+                const int ts = G.tile_shift, ti = i >> ts;
+                RedEntry r;
+                r.base = G.scratch_off + ((long long)(ti * (ti + 1) / 2) << (2 * ts)) + ((i & ((1 << ts) - 1)) << ts);
+                r.rows = G.ao_begin;
+                r.li_ts = i | (ts << 16);
+                hgg[fill[hgrows[G.ao_begin + i].x]++] = r;
+            }
     }
 
     // LDS budget: per class and mode, 32-point tiles when they fit, else 16.
@@ -478,7 +484,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     }
 
     if (P.n_group_list > 0 && vgroup_lds_bytes(false, P.max_group_nsh, P.max_group_nprim) > P.lds_limit) return kLimit;
-    const size_t need = sizeof(double) * (size_t)scratch + sizeof(int) * hloc.size() +
+    P.reduce_waves = reduce_v_waves(n_ao, P.lds_limit);
+    if (P.reduce_waves < 1) return kLimit;
+    const size_t need = sizeof(double) * (size_t)scratch + sizeof(RedEntry) * hgg.size() +
                         sizeof(double) * (size_t)n_ao * n_ao * 2 + sizeof(float) * (size_t)n_ao * n_ao +
                         sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
     if (mem_limit > 0 && need > mem_limit) return kLimit;
@@ -490,7 +498,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          upload(P.tiles32, t32) &&
          upload(P.tiles16, t16) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
-         upload(P.loc, hloc) && upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
+         upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) &&
          P.npart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) && P.en.alloc(sizeof(double) * 2) &&
@@ -521,9 +529,10 @@ void mark(Plan& P, hipStream_t s) {
 
 void resolve_profile(Plan& P) {
     auto& ev = P.prof.ev;
-    for (size_t i = 0; i + 5 <= ev.size(); i += 5) {
-        hipEventSynchronize(ev[i + 4]);
-        for (int k = 0; k < 4; ++k) {
+    constexpr int K = Profile::kMarks;
+    for (size_t i = 0; i + K <= ev.size(); i += K) {
+        hipEventSynchronize(ev[i + K - 1]);
+        for (int k = 0; k < K - 1; ++k) {
             float ms = 0.f;
             if (hipEventElapsedTime(&ms, ev[i + k], ev[i + k + 1]) == hipSuccess) P.prof.ms[k] += ms;
         }
@@ -540,6 +549,7 @@ int run_rho(Plan& P, int mode, bool caller_order, const double* dP, hipStream_t 
     GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), dP, P.n_ao, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
     if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
+    if (!caller_order) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
     for (size_t c = 0; c < P.classes[0].size(); ++c) {
@@ -559,23 +569,18 @@ int run_rho(Plan& P, int mode, bool caller_order, const double* dP, hipStream_t 
 
 // V stage: register-resident kernel over its groups, tile-pair kernel over
 // the rest, then the deterministic reduction into the full symmetric V.
-int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s) {
+// With fused set this is part of a fused build and records its profiling mark.
+int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s, bool fused) {
     VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
              P.group_sh.as<int2>(), wv, P.perm.as<int>(), wv_perm, P.scratch.as<double>()};
     if (launch_vxc_group(va, f32, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) != hipSuccess)
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
-    mark(P, s);
-    const long long nn = (long long)P.n_ao * P.n_ao;
-    ReduceArgs rd{P.loc.as<int>(), P.ao_gbegin.as<int>(), P.ao_groups.as<int>(), P.groups.as<VGroup>(),
-                  P.n_groups, P.n_ao, P.scratch.as<double>(), dV};
-    if (P.n_groups == 0) {
-        if (launch_fill_zero(dV, nn, s) != hipSuccess) return kLaunchFail;
-    } else if (launch_reduce_v(rd, s) != hipSuccess) {
-        return kLaunchFail;
-    }
-    return kOk;
+    if (fused) mark(P, s);
+    ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_rows.as<int2>(), P.n_ao,
+                  P.scratch.as<double>(), dV};
+    return launch_reduce_v(rd, P.reduce_waves, s) == hipSuccess ? kOk : kLaunchFail;
 }
 
 // Fused RKS build on device pointers, asynchronous on stream s.
@@ -587,11 +592,12 @@ int run_build(Plan& P, int func, int prec, const double* dP, double* dV, double*
     mark(P, s);
     const int rc = run_rho(P, f32 ? kRhoMixed : kRhoFp64, false, dP, s);
     if (rc != kOk) return rc;
+    mark(P, s);
     XcArgs xa{P.rho_s.as<double4>(), P.pts.as<double4>(), P.n_pts, func, P.wv.as<double4>(), P.epart.as<double>(),
               P.npart.as<double>()};
     if (launch_xc(xa, s) != hipSuccess) return kLaunchFail;
     mark(P, s);
-    const int rv = run_v(P, f32, P.wv.as<double4>(), 0, dV, s);
+    const int rv = run_v(P, f32, P.wv.as<double4>(), 0, dV, s, true);
     if (rv != kOk) return rv;
     mark(P, s);
     if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
@@ -679,7 +685,7 @@ int calchem_hip_vxc_accum(uint64_t grid_id, const double* wv_in, double* vxc_out
     const size_t nn = (size_t)n_ao * n_ao;
     if (hipMemcpyAsync(P.wv_caller.p, wv_in, sizeof(double4) * (size_t)n_pts, hipMemcpyHostToDevice, s) != hipSuccess)
         return kLaunchFail;
-    const int rv = run_v(P, false, P.wv_caller.as<double4>(), 1, P.V.as<double>(), s);
+    const int rv = run_v(P, false, P.wv_caller.as<double4>(), 1, P.V.as<double>(), s, false);
     if (rv != kOk) return rv;
     if (hipMemcpyAsync(vxc_out, P.V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s) != hipSuccess)
         return kLaunchFail;
@@ -796,10 +802,12 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats* out) {
     out->device_bytes = P.device_bytes();
     out->lds_bytes_per_block = P.lds_limit;
     out->profiled_builds = P.prof.builds;
-    out->ms_convert_and_rho = P.prof.ms[0];
-    out->ms_vxc = P.prof.ms[1];
-    out->ms_reduce_v = P.prof.ms[2];
-    out->ms_reduce_en = P.prof.ms[3];
+    out->ms_gather = P.prof.ms[0];
+    out->ms_rho = P.prof.ms[1];
+    out->ms_xc = P.prof.ms[2];
+    out->ms_vxc = P.prof.ms[3];
+    out->ms_reduce_v = P.prof.ms[4];
+    out->ms_reduce_en = P.prof.ms[5];
     return kOk;
 }
 

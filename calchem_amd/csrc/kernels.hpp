@@ -64,12 +64,19 @@ struct GatherArgs {
     int out64_leading_only; // write out64 only for the block's first m64 columns
 };
 
+// One (AO mu, group holding mu) entry of the V reduction: where row li(mu)
+// of the group's lower tiles starts in scratch, the group's row list, and
+// li | tile_shift << 16.
+struct RedEntry {
+    long long base;
+    int rows;
+    int li_ts;
+};
+
 struct ReduceArgs {
-    const int* loc;         // [group][n_ao] group-local row of each AO, or -1
-    const int* ao_gbegin;   // [n_ao + 1] CSR of the groups holding each AO
-    const int* ao_groups;   // group indices, ascending per AO
-    const VGroup* groups;
-    int n_groups;
+    const int* ao_gbegin;   // [n_ao + 1] CSR of the entries of each AO, groups ascending
+    const RedEntry* entries;
+    const int2* group_rows; // per group row: (global AO, group-local shell)
     int n_ao;
     const double* scratch;
     double* V;              // n_ao x n_ao, full symmetric
@@ -96,7 +103,8 @@ hipError_t launch_xc(const XcArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);
 size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim);
 hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh, int max_prim, hipStream_t s);
-hipError_t launch_reduce_v(const ReduceArgs& a, hipStream_t s);
+int reduce_v_waves(int n_ao, size_t lds_limit);
+hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s);
 hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s);
 hipError_t launch_to_float(const double* x, float* y, long long n, hipStream_t s);
 hipError_t launch_fill_zero(double* x, long long n, hipStream_t s);

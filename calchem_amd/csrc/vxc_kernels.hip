@@ -17,6 +17,8 @@
 // T = double runs FP64 MFMA (v_mfma_f64_16x16x4f64); T = float runs FP32 MFMA
 // (v_mfma_f32_16x16x4f32) with FP32 accumulation inside a tile and FP64 for
 // the functional, the per-point reductions and every sum across tiles.
+#include <algorithm>
+
 #include "ao_device.hpp"
 #include "common.hpp"
 #include "kernels.hpp"
@@ -87,11 +89,23 @@ __host__ __device__ constexpr size_t rho_lds_fixed() {
 // (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
 __device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >> 2); }
 
+// Workgroups are dispatched round robin over the 8 XCDs (workgroup i to XCD
+// i mod 8), each with its own L2.  This maps workgroup i to work item
+// xcd_order(i, n) so that XCD x walks one contiguous range of the work list:
+// consecutive tiles of a block (which read the same density block) then share
+// an L2.  A bijection of [0, n).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_order(int i, int n) {
+    const int x = i % kXcds, k = i / kXcds;
+    const int q = n / kXcds, r = n % kXcds;
+    return x * q + min(x, r) + k;
+}
+
 template <typename TS, int PTS, bool HYB>
 __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int LDP = rho_ldp<PTS>();
-    const RhoTile tile = a.tiles[blockIdx.x];
+    const RhoTile tile = a.tiles[xcd_order(blockIdx.x, gridDim.x)];
     const RhoBlock blk = a.blocks[tile.block];
     const int m = blk.m, mp = blk.m_pad, np = tile.n_pts, nsh = blk.n_sh;
     const int m64 = HYB ? blk.m64 : 0;
@@ -512,42 +526,39 @@ __global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
     }
 }
 
-// V[mu][nu] (mu >= nu) = sum over the groups holding mu, in ascending group
-// order, of that group's tile element; groups without nu add nothing.  Four
-// groups are looked up per round so their loads overlap.
-__device__ __forceinline__ double group_elem(const ReduceArgs& a, int g, int mu, int nu) {
-    const int n = a.n_ao;
-    const int lj = a.loc[static_cast<long long>(g) * n + nu];
-    if (lj < 0) return 0.0;
-    const int li = a.loc[static_cast<long long>(g) * n + mu];
-    const VGroup G = a.groups[g];
-    const int ts = G.tile_shift, mask = (1 << ts) - 1;
-    const int ti = li >> ts, tj = lj >> ts;
-    return a.scratch[G.scratch_off + (static_cast<long long>(ti * (ti + 1) / 2 + tj) << (2 * ts)) +
-                     ((li & mask) << ts) + (lj & mask)];
-}
-
-__global__ __launch_bounds__(kThreads) void k_reduce_v(ReduceArgs a) {
-    const int mu = blockIdx.y;
-    const int nu = blockIdx.x * kThreads + threadIdx.x;
-    if (nu > mu || nu >= a.n_ao) return;
-    const int n = a.n_ao;
-    const int e0 = a.ao_gbegin[mu], e1 = a.ao_gbegin[mu + 1];
-    double s = 0.0;
-    int e = e0;
-    for (; e + 4 <= e1; e += 4) {
-        const double v0 = group_elem(a, a.ao_groups[e], mu, nu);
-        const double v1 = group_elem(a, a.ao_groups[e + 1], mu, nu);
-        const double v2 = group_elem(a, a.ao_groups[e + 2], mu, nu);
-        const double v3 = group_elem(a, a.ao_groups[e + 3], mu, nu);
-        s += v0;
-        s += v1;
-        s += v2;
-        s += v3;
+// One workgroup per AO row mu: V[mu][nu] (nu <= mu) = sum over the groups
+// holding mu of row li(mu) of that group's lower tiles, scattered through the
+// group's row list.  Wave w of the workgroup takes entries w, w + W, ... in
+// ascending group order into its own LDS copy of the row (a wave's LDS
+// operations execute in order, so its lanes need no barrier between groups);
+// the W copies are then added in wave order, the row written to V and
+// mirrored (V is exactly symmetric).  Group rows are in ascending AO order, so
+// nu <= mu is lj <= li.  Deterministic for a given plan.
+__global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
+    extern __shared__ double srow[];  // [W][mu + 1]
+    const int mu = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int nw = blockDim.x >> 6, len = mu + 1;
+    for (int j = tid; j < nw * len; j += blockDim.x) srow[j] = 0.0;
+    __syncthreads();
+    double* __restrict__ my = srow + w * len;
+    const int e1 = a.ao_gbegin[mu + 1];
+    for (int e = a.ao_gbegin[mu] + w; e < e1; e += nw) {
+        const RedEntry r = a.entries[e];
+        const int li = r.li_ts & 0xffff, ts = r.li_ts >> 16, mask = (1 << ts) - 1;
+        const double* __restrict__ base = a.scratch + r.base;
+        const int2* __restrict__ rows = a.group_rows + r.rows;
+        for (int lj = lane; lj <= li; lj += 64)
+            my[rows[lj].x] += base[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
     }
-    for (; e < e1; ++e) s += group_elem(a, a.ao_groups[e], mu, nu);
-    a.V[static_cast<long long>(mu) * n + nu] = s;
-    a.V[static_cast<long long>(nu) * n + mu] = s;
+    __syncthreads();
+    const long long n = a.n_ao;
+    for (int j = tid; j < len; j += blockDim.x) {
+        double v = 0.0;
+        for (int c = 0; c < nw; ++c) v += srow[c * len + j];
+        a.V[mu * n + j] = v;
+        a.V[j * n + mu] = v;
+    }
 }
 
 __global__ __launch_bounds__(1024) void k_reduce_en(const double* __restrict__ e, const double* __restrict__ ne,
@@ -707,10 +718,27 @@ hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh
                 : launch_vxc_group_t<double>(a, n_groups, max_nsh, max_prim, s);
 }
 
-hipError_t launch_reduce_v(const ReduceArgs& a, hipStream_t s) {
+// Waves (row copies) per reduction workgroup that fit in LDS: up to 16.
+int reduce_v_waves(int n_ao, size_t lds_limit) {
+    const size_t row = sizeof(double) * (size_t)std::max(n_ao, 1);
+    return (int)std::min<size_t>(16, lds_limit / row);
+}
+
+hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s) {
     if (a.n_ao <= 0) return hipSuccess;
-    dim3 grid((a.n_ao + kThreads - 1) / kThreads, a.n_ao);
-    hipLaunchKernelGGL(k_reduce_v, grid, dim3(kThreads), 0, s, a);
+    static int attr_set = 0;
+    if (!attr_set) {
+        int dev = 0, lim = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_reduce_v),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+        if (e != hipSuccess) return e;
+        attr_set = 1;
+    }
+    // Row mu needs (mu + 1) doubles per wave; size for the last row.
+    hipLaunchKernelGGL(k_reduce_v, dim3(a.n_ao), dim3(64 * waves), sizeof(double) * (size_t)a.n_ao * waves, s, a);
     return hipGetLastError();
 }
 

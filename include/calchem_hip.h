@@ -92,7 +92,9 @@ typedef struct {
     size_t device_bytes;
     size_t lds_bytes_per_block;      /* device limit used to size the rho tiles */
     int64_t profiled_builds;         /* builds timed since profiling was switched on */
-    double ms_convert_and_rho;       /* summed HIP-event times of those builds */
+    double ms_gather;                /* summed HIP-event times of those builds, per stage: */
+    double ms_rho;                   /*   density-block gather, rho kernel(s), functional, */
+    double ms_xc;                    /*   V kernel(s), V reduction, E_xc / N reduction */
     double ms_vxc;
     double ms_reduce_v;
     double ms_reduce_en;
