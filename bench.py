@@ -149,6 +149,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block-points", type=int, default=0)
     ap.add_argument("--group-points", type=int, default=0)
+    ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16, 32])
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -175,7 +176,8 @@ def main() -> int:
     log(f"{args.workload}: {mol.n_atoms} atoms, {basis.n_ao} AO, {grid.n_points} points "
         f"(setup {time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
-    eng = RksVxcEngine(basis, grid, args.block_points, args.group_points)
+    eng = RksVxcEngine(basis, grid, args.block_points, args.group_points,
+                       rho_tile_points=args.rho_tile_points)
     t_plan = time.perf_counter() - t0
     st0 = eng.stats()
     log(f"plan {t_plan:.2f}s: {st0['n_blocks']} blocks, {st0['n_groups']} groups, rho tiles "

@@ -29,7 +29,7 @@ class CalchemHipError(RuntimeError):
 class VxcOptions(C.Structure):
     _fields_ = [("block_points", C.c_int32), ("group_points", C.c_int32),
                 ("memory_limit_bytes", C.c_size_t), ("mixed_exact_threshold", C.c_double),
-                ("reserved", C.c_int32 * 4)]
+                ("rho_tile_points", C.c_int32), ("reserved", C.c_int32 * 3)]
 
 
 class VxcStats(C.Structure):

@@ -21,6 +21,8 @@ HIPCC_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # keep the compiler from emitting packed FP32 VALU ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # keep MFMA accumulators in ordinary VGPRs (no AGPR copies around loops)
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-value", "-Wno-unused-result", "-Wno-pass-failed",
 ]
 

@@ -13,13 +13,11 @@
 namespace chx {
 
 __device__ __forceinline__ void mono_d(double x, int l, double& p, double& d) {
-    // x^l and d/dx x^l for l <= 3 (the reference's pow_d0_d1_d2 branches).
-    switch (l) {
-    case 0: p = 1.0; d = 0.0; break;
-    case 1: p = x; d = 1.0; break;
-    case 2: p = x * x; d = 2.0 * x; break;
-    default: p = x * x * x; d = 3.0 * x * x; break;
-    }
+    // x^l and d/dx x^l for l <= 3 (the reference's pow_d0_d1_d2 cases),
+    // by selects so that lanes with different l do not diverge.
+    const double x2 = x * x;
+    p = l == 0 ? 1.0 : (l == 1 ? x : (l == 2 ? x2 : x2 * x));
+    d = l == 0 ? 0.0 : (l == 1 ? 1.0 : (l == 2 ? 2.0 * x : 3.0 * x * x));
 }
 
 struct AoVal {
@@ -114,6 +112,67 @@ __device__ __forceinline__ AoVal ao_from_radial_row(const DevShell& s, const Row
     ao.ly = (r.lmn >> 2) & 3;
     ao.lz = (r.lmn >> 4) & 3;
     return ao_from_radial(s, ao, x, y, z, rr);
+}
+
+// FP32 variants for the mixed build's V stage.  Offsets from the shell
+// centre are formed in FP64 and rounded once, so r^2 carries FP32 relative
+// error only; e^{-x} then has absolute error below ~1e-7 x e^{-x} <= 4e-8.
+// e^{-x} uses the hardware base-2 exponential (v_exp_f32).
+struct AoValF {
+    float v, gx, gy, gz;
+};
+
+__device__ __forceinline__ float exp_neg_f(float x) { return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f); }
+
+__device__ __forceinline__ void mono_df(float x, int l, float& p, float& d) {
+    const float x2 = x * x;
+    p = l == 0 ? 1.0f : (l == 1 ? x : (l == 2 ? x2 : x2 * x));
+    d = l == 0 ? 0.0f : (l == 1 ? 1.0f : (l == 2 ? 2.0f * x : 3.0f * x2));
+}
+
+// Radial sums in FP32 (R0, R1), with the same cutoff and 1e-18 rules.
+__device__ __forceinline__ float2 shell_radial_f(const DevShell& s, const double2* __restrict__ prims, double x,
+                                                 double y, double z) {
+    const float dx = static_cast<float>(x - s.cx), dy = static_cast<float>(y - s.cy),
+                dz = static_cast<float>(z - s.cz);
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    float r0 = 0.0f, r1 = 0.0f;
+    if (r2 <= static_cast<float>(s.r2cut)) {
+        for (int p = 0; p < s.n_prims; ++p) {
+            const double2 pr = prims[s.prim_start + p];
+            const float al = static_cast<float>(pr.x);
+            const float e = static_cast<float>(pr.y) * exp_neg_f(al * r2);
+            r0 += e;
+            r1 -= 2.0f * al * e;
+        }
+        if (fabsf(r0) < 1e-18f && fabsf(r1) < 1e-18f) r0 = r1 = 0.0f;
+    }
+    return make_float2(r0, r1);
+}
+
+__device__ __forceinline__ AoValF ao_from_radial_row_f(const DevShell& s, const RowL& r, double x, double y,
+                                                       double z, float2 rr) {
+    AoValF o{0.0f, 0.0f, 0.0f, 0.0f};
+    if (rr.x == 0.0f && rr.y == 0.0f) return o;
+    const float dx = static_cast<float>(x - s.cx), dy = static_cast<float>(y - s.cy),
+                dz = static_cast<float>(z - s.cz);
+    float px, dpx, py, dpy, pz, dpz;
+    mono_df(dx, r.lmn & 3, px, dpx);
+    mono_df(dy, (r.lmn >> 2) & 3, py, dpy);
+    mono_df(dz, (r.lmn >> 4) & 3, pz, dpz);
+    const float m = px * py * pz;
+    const float sc = static_cast<float>(r.scale);
+    const float v = sc * m * rr.x;
+    const float gx = sc * (dpx * py * pz * rr.x + m * dx * rr.y);
+    const float gy = sc * (px * dpy * pz * rr.x + m * dy * rr.y);
+    const float gz = sc * (px * py * dpz * rr.x + m * dz * rr.y);
+    if (fabsf(v) > 1e-16f || fabsf(gx) > 1e-16f || fabsf(gy) > 1e-16f || fabsf(gz) > 1e-16f) {
+        o.v = v;
+        o.gx = gx;
+        o.gy = gy;
+        o.gz = gz;
+    }
+    return o;
 }
 
 // Stage the rows, shells (with block-local primitive offsets) and primitives

@@ -200,6 +200,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     const int nb_v = (opt && opt->group_points > 0) ? opt->group_points : 1024;
     const size_t mem_limit = opt ? opt->memory_limit_bytes : 0;
     if (opt && opt->mixed_exact_threshold > 0.0) P.mixed_threshold = opt->mixed_exact_threshold;
+    const int want_pts = opt ? opt->rho_tile_points : 0;
+    if (want_pts != 0 && want_pts != 16 && want_pts != 32) return kBadArgs;
 
     // Device shell / AO tables.
     std::vector<DevShell> hs(n_shells);
@@ -478,7 +480,10 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (int mode = 0; mode < 2; ++mode) {
         P.class_pts[mode].assign(P.classes[0].size(), 0);
         for (size_t c = 0; c < P.classes[0].size(); ++c) {
-            if (rho_lds_bytes(mode, 32, P.classes[0][c]) <= P.lds_limit) P.class_pts[mode][c] = 32;
+            // 16-point tiles by default: their smaller LDS footprint lets two
+            // workgroups share a CU, which outweighs reading each density
+            // block twice as often.
+            if (want_pts == 32 && rho_lds_bytes(mode, 32, P.classes[0][c]) <= P.lds_limit) P.class_pts[mode][c] = 32;
             else if (rho_lds_bytes(mode, 16, P.classes[1][c]) <= P.lds_limit) P.class_pts[mode][c] = 16;
         }
     }
@@ -505,8 +510,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          P.P64.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.P32.alloc(sizeof(float) * (size_t)n_ao * n_ao) &&
          P.V.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.rho.alloc(sizeof(double4) * (size_t)n_pts) &&
          P.rho_s.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
-         P.psub64.alloc(sizeof(double) * (size_t)std::max<long long>(P.psub_elems, 1)) &&
-         P.psub32.alloc(sizeof(float) * (size_t)std::max<long long>(P.psub_elems, 1));
+         P.psub64.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)) &&
+         P.psub32.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail));
     if (!ok) return kAllocFail;
     // Points of blocks with no significant AO are never written by the rho
     // kernel: their sorted rho stays zero for the life of the plan.

@@ -55,11 +55,11 @@ template <typename T>
 struct Vec4;
 template <>
 struct Vec4<double> {
-    using type = double4;
+    using type = v4d;
 };
 template <>
 struct Vec4<float> {
-    using type = float4;
+    using type = v4f;
 };
 
 // MFMA 16x16x4 operand layout (one value per lane): A[i][k] at lane i + 16 k,
@@ -83,6 +83,13 @@ __host__ __device__ constexpr int rho_ldp() {
 template <int PTS>
 __host__ __device__ constexpr size_t rho_lds_fixed() {
     return sizeof(double4) * PTS + sizeof(double) * 16 * PTS;
+}
+
+// x unchanged, through a register the compiler cannot see into.
+template <typename V>
+__device__ __forceinline__ V opaque(V x) {
+    asm("" : "+v"(x));
+    return x;
 }
 
 // Position in an f64 16x16x4 A tile that makes its result element v of lane
@@ -175,9 +182,21 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
         if constexpr (HYB) {
 #pragma unroll
             for (int h = 0; h < NH; ++h) acc64[h] = Mma<double>::zero();
-            // Leading rows (large |phi|) in FP64 with the FP64 density block.
+            // Leading rows (large |phi|) in FP64 with the FP64 density block,
+            // four k-steps per round with their loads issued together.
             const double* __restrict__ Arow = a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
-            for (int k = 0; k < m64; k += 4) {
+            int k = 0;
+            for (; k + 16 <= m64; k += 16) {
+                double av[4];
+#pragma unroll
+                for (int s = 0; s < 4; ++s) av[s] = Arow[k + 4 * s + kq];
+#pragma unroll
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int h = 0; h < NH; ++h)
+                        acc64[h] = Mma<double>::op(av[s], sPhi64[(k + 4 * s + kq) * LDP + h * 16 + col], acc64[h]);
+            }
+            for (; k < m64; k += 4) {
                 const int kk = k + kq;
                 const double av = Arow[kk];
 #pragma unroll
@@ -189,6 +208,8 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
             // K is visited in chunks of 16 whose slots are permuted so that
             // lane (i, q) reads four consecutive density entries per load:
             // step s of a chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].
+            // Density loads run two chunks ahead (reads past the row end land
+            // in the next row or the buffer tail and are not used).
             const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>(i0 + col) * mp;
             int k = m64;
             for (; k < mp && (k & 15); k += 4) {
@@ -197,22 +218,30 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
 #pragma unroll
                 for (int h = 0; h < NH; ++h) acc[h] = M::op(av, sPhi[kk * LDP + h * 16 + col], acc[h]);
             }
+            // The first two chunks enter the rotation through an empty asm:
+            // with a non-load among the incoming values the compiler cannot
+            // fold the rotated registers back into one load at the point of
+            // use, so each load is issued two chunks before it is consumed.
             using V4 = typename Vec4<TS>::type;
-            if (k < mp) {
-                V4 cur = *reinterpret_cast<const V4*>(Arow + k + 4 * kq);
-                for (; k < mp; k += 16) {
-                    V4 nxt = cur;
-                    if (k + 16 < mp) nxt = *reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq);
-                    const int kb = k + 4 * kq;
+            V4 a0 = opaque(*reinterpret_cast<const V4*>(Arow + k + 4 * kq));
+            V4 a1 = opaque(*reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq));
+            for (; k < mp; k += 16) {
+                const V4 a2 = *reinterpret_cast<const V4*>(Arow + k + 32 + 4 * kq);
+                const int kb = k + 4 * kq;
+                TS b[4][NH];
 #pragma unroll
-                    for (int h = 0; h < NH; ++h) {
-                        acc[h] = M::op(cur.x, sPhi[(kb + 0) * LDP + h * 16 + col], acc[h]);
-                        acc[h] = M::op(cur.y, sPhi[(kb + 1) * LDP + h * 16 + col], acc[h]);
-                        acc[h] = M::op(cur.z, sPhi[(kb + 2) * LDP + h * 16 + col], acc[h]);
-                        acc[h] = M::op(cur.w, sPhi[(kb + 3) * LDP + h * 16 + col], acc[h]);
-                    }
-                    cur = nxt;
+                for (int s = 0; s < 4; ++s)
+#pragma unroll
+                    for (int h = 0; h < NH; ++h) b[s][h] = sPhi[(kb + s) * LDP + h * 16 + col];
+#pragma unroll
+                for (int h = 0; h < NH; ++h) {
+                    acc[h] = M::op(a0[0], b[0][h], acc[h]);
+                    acc[h] = M::op(a0[1], b[1][h], acc[h]);
+                    acc[h] = M::op(a0[2], b[2][h], acc[h]);
+                    acc[h] = M::op(a0[3], b[3][h], acc[h]);
                 }
+                a0 = a1;
+                a1 = a2;
             }
         }
         // Epilogue: this lane holds T[i0 + row(kq, v)][h*16 + col].
@@ -407,23 +436,46 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
             }
 }
 
-// Register-resident V kernel: one 512-thread workgroup per group.  Every
-// lower pair of 16-row tiles of the group's V lives in the accumulators of
-// one wavefront for the whole pass over the group's points; 16 points at a
+// Register-resident V kernel: one workgroup of NW wavefronts per group.
+// Every lower pair of 16-row tiles of the group's V lives in the accumulators
+// of one wavefront for the whole pass over the group's points; 16 points at a
 // time, radial sums (per shell) then phi and Z = w v_rho/2 phi + wu . dphi
 // (per row) are staged in LDS, each evaluated once, and every pair takes
-// acc += Z_I Phi_J^T + Phi_I Z_J^T.
-constexpr int kPairsPerWave = (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + 7) / 8;
+// acc += Z_I Phi_J^T + Phi_I Z_J^T.  FP64: 8 waves, AO evaluation in FP64.
+// FP32 (mixed build): 16 waves (half the pairs per wave, four waves per SIMD)
+// and AO evaluation in FP32.
+template <typename T>
+struct VGroupCfg;
+template <>
+struct VGroupCfg<double> {
+    static constexpr int kWaves = 8;
+    using Rad = double2;
+};
+template <>
+struct VGroupCfg<float> {
+    static constexpr int kWaves = 16;
+    using Rad = float2;
+};
+
+template <typename T>
+__host__ __device__ constexpr int vgroup_pairs_per_wave() {
+    return (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + VGroupCfg<T>::kWaves - 1) / VGroupCfg<T>::kWaves;
+}
+
+// Pair table entries, rounded up to keep 16-byte alignment after it.
+constexpr int kGroupMaxPairsPad = (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + 1) & ~1;
 
 template <typename T>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
-    return sizeof(double4) * 2 * kVPts + sizeof(RowL) * kGroupMaxTiles * 16 +
+    return sizeof(double4) * 2 * kVPts + sizeof(int2) * kGroupMaxPairsPad + sizeof(RowL) * kGroupMaxTiles * 16 +
            sizeof(T) * 2 * kGroupMaxTiles * 16 * (kVPts + 1);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
+__global__ __launch_bounds__(VGroupCfg<T>::kWaves * 64) void k_vxc_group(VArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int NW = VGroupCfg<T>::kWaves, NT = NW * 64, PPW = vgroup_pairs_per_wave<T>();
+    using Rad = typename VGroupCfg<T>::Rad;
     constexpr int LD = kVPts + 1;
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh;
@@ -434,14 +486,16 @@ __global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
     q += sizeof(double4) * kVPts;
     double4* sWv = reinterpret_cast<double4*>(q);
     q += sizeof(double4) * kVPts;
+    int2* sPair = reinterpret_cast<int2*>(q);
+    q += sizeof(int2) * kGroupMaxPairsPad;
     RowL* sRow = reinterpret_cast<RowL*>(q);
     q += sizeof(RowL) * kGroupMaxTiles * 16;
     T* sP = reinterpret_cast<T*>(q);
     q += sizeof(T) * kGroupMaxTiles * 16 * LD;
     T* sZ = reinterpret_cast<T*>(q);
     q += sizeof(T) * kGroupMaxTiles * 16 * LD;
-    double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][kVPts]
-    q += sizeof(double2) * nsh * kVPts;
+    Rad* sRad = reinterpret_cast<Rad*>(q);  // [nsh][kVPts]
+    q += sizeof(Rad) * ((nsh * kVPts + 1) & ~1);
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
     q += sizeof(DevShell) * nsh;
     double2* sPrim = reinterpret_cast<double2*>(q);
@@ -450,20 +504,20 @@ __global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int col = lane & 15, kq = lane >> 4;
     stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
-               sPrim, tid, kGroupThreads);
+               sPrim, tid, NT);
 
-    using M = Mma<T>;
-    typename M::acc acc[kPairsPerWave];
-    int ptij[kPairsPerWave];  // ti << 16 | tj
-#pragma unroll
-    for (int w = 0; w < kPairsPerWave; ++w) {
-        acc[w] = M::zero();
-        // pair index p = ti (ti + 1) / 2 + tj, handed out round robin to the waves
-        const int p = wave + 8 * w;
+    // Pair p = ti (ti + 1) / 2 + tj (ti >= tj) goes to wave p mod NW; the
+    // LDS table holds the LDS row offsets of tiles ti and tj (in elements).
+    for (int p = tid; p < npairs; p += NT) {
         int ti = 0;
         while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
-        ptij[w] = (ti << 16) | (p - ti * (ti + 1) / 2);
+        const int tj = p - ti * (ti + 1) / 2;
+        sPair[p] = make_int2(ti * 16 * LD, tj * 16 * LD);
     }
+    using M = Mma<T>;
+    typename M::acc acc[PPW];
+#pragma unroll
+    for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
 
     for (int q0 = 0; q0 < G.n_pts; q0 += kVPts) {
         __syncthreads();
@@ -479,35 +533,44 @@ __global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
             }
         }
         __syncthreads();
-        for (int idx = tid; idx < nsh * kVPts; idx += kGroupThreads) {
+        for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
-            double2 rr = make_double2(0.0, 0.0);
-            if (q0 + p < G.n_pts) {
-                const double4 x = sPt[p];
-                rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
+            const double4 x = sPt[p];
+            const bool live = q0 + p < G.n_pts;
+            if constexpr (sizeof(T) == 8) {
+                sRad[idx] = live ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
+            } else {
+                sRad[idx] = live ? shell_radial_f(sSh[sh], sPrim, x.x, x.y, x.z) : make_float2(0.0f, 0.0f);
             }
-            sRad[idx] = rr;
         }
         __syncthreads();
-        for (int idx = tid; idx < mp * kVPts; idx += kGroupThreads) {
+        for (int idx = tid; idx < mp * kVPts; idx += NT) {
             const int r = idx / kVPts, p = idx % kVPts;
             const RowL row = sRow[r];
-            double v = 0.0, z = 0.0;
+            T v = 0, z = 0;
             if (row.sh >= 0) {
                 const double4 x = sPt[p];
-                const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
                 const double4 w = sWv[p];
-                v = f.v;
-                z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
+                if constexpr (sizeof(T) == 8) {
+                    const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
+                    v = f.v;
+                    z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
+                } else {
+                    const AoValF f = ao_from_radial_row_f(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
+                    v = f.v;
+                    z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx +
+                        static_cast<float>(w.z) * f.gy + static_cast<float>(w.w) * f.gz;
+                }
             }
-            sP[r * LD + p] = static_cast<T>(v);
-            sZ[r * LD + p] = static_cast<T>(z);
+            sP[r * LD + p] = v;
+            sZ[r * LD + p] = z;
         }
         __syncthreads();
 #pragma unroll
-        for (int w = 0; w < kPairsPerWave; ++w) {
-            if (wave + 8 * w >= npairs) continue;
-            const int ri = ((ptij[w] >> 16) * 16 + col) * LD, rj = ((ptij[w] & 0xffff) * 16 + col) * LD;
+        for (int w = 0; w < PPW; ++w) {
+            if (wave + NW * w >= npairs) continue;
+            const int2 pr = sPair[wave + NW * w];
+            const int ri = pr.x + col * LD, rj = pr.y + col * LD;
 #pragma unroll
             for (int k = 0; k < kVPts; k += 4) {
                 const int kk = k + kq;
@@ -517,8 +580,8 @@ __global__ __launch_bounds__(kGroupThreads) void k_vxc_group(VArgs a) {
         }
     }
 #pragma unroll
-    for (int w = 0; w < kPairsPerWave; ++w) {
-        const int p = wave + 8 * w;
+    for (int w = 0; w < PPW; ++w) {
+        const int p = wave + NW * w;
         if (p >= npairs) continue;
         double* out = a.scratch + G.scratch_off + static_cast<long long>(p) * 256;
 #pragma unroll
@@ -691,7 +754,9 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
 
 size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim) {
     const size_t fixed = fp32 ? vgroup_lds_fixed<float>() : vgroup_lds_fixed<double>();
-    return fixed + (sizeof(double2) * kVPts + sizeof(DevShell)) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
+    const size_t rad = fp32 ? sizeof(float2) * (size_t)((max_nsh * kVPts + 1) & ~1)
+                            : sizeof(double2) * (size_t)max_nsh * kVPts;
+    return fixed + rad + sizeof(DevShell) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
 }
 
 template <typename T>
@@ -708,7 +773,7 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
         attr_set = 1;
     }
     if (n_groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vxc_group<T>, dim3(n_groups), dim3(kGroupThreads),
+    hipLaunchKernelGGL(k_vxc_group<T>, dim3(n_groups), dim3(VGroupCfg<T>::kWaves * 64),
                        vgroup_lds_bytes(sizeof(T) == 4, max_nsh, max_prim), s, a);
     return hipGetLastError();
 }

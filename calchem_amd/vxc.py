@@ -39,7 +39,8 @@ def _records(basis: BasisSet):
 
 class RksVxcEngine:
     def __init__(self, basis: BasisSet, grid: MolecularGrid, block_points: int = 0,
-                 group_points: int = 0, memory_limit_bytes: int = 0, mixed_exact_threshold: float = 0.0):
+                 group_points: int = 0, memory_limit_bytes: int = 0, mixed_exact_threshold: float = 0.0,
+                 rho_tile_points: int = 0):
         self._lib = _native.require_device()
         self.n_ao = basis.n_ao
         self.n_points = grid.n_points
@@ -47,7 +48,8 @@ class RksVxcEngine:
         self._kernel_invocations = 0
         shells, prims, comps = _records(basis)
         xyzw = grid.xyzw()
-        opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold)
+        opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold,
+                                  rho_tile_points)
         h = _native.new_handle()
         rc = self._lib.calchem_hip_vxc_create(
             _native.ptr(shells), len(shells), _native.ptr(prims), len(prims), _native.ptr(comps),
