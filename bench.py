@@ -150,6 +150,7 @@ def main() -> int:
     ap.add_argument("--block-points", type=int, default=0)
     ap.add_argument("--group-points", type=int, default=0)
     ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16, 32])
+    ap.add_argument("--group-max-rows", type=int, default=0, choices=[0, 192, 272])
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -177,7 +178,7 @@ def main() -> int:
         f"(setup {time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     eng = RksVxcEngine(basis, grid, args.block_points, args.group_points,
-                       rho_tile_points=args.rho_tile_points)
+                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows)
     t_plan = time.perf_counter() - t0
     st0 = eng.stats()
     log(f"plan {t_plan:.2f}s: {st0['n_blocks']} blocks, {st0['n_groups']} groups, rho tiles "

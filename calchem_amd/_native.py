@@ -29,7 +29,7 @@ class CalchemHipError(RuntimeError):
 class VxcOptions(C.Structure):
     _fields_ = [("block_points", C.c_int32), ("group_points", C.c_int32),
                 ("memory_limit_bytes", C.c_size_t), ("mixed_exact_threshold", C.c_double),
-                ("rho_tile_points", C.c_int32), ("reserved", C.c_int32 * 3)]
+                ("rho_tile_points", C.c_int32), ("group_max_rows", C.c_int32), ("reserved", C.c_int32 * 2)]
 
 
 class VxcStats(C.Structure):
@@ -53,7 +53,8 @@ EXPORTS = [
     "calchem_hip_is_available", "calchem_hip_get_memory_info", "calchem_hip_total_allocated_bytes",
     "calchem_hip_vxc_upload_grid", "calchem_hip_vxc_eval_rho", "calchem_hip_vxc_accum",
     "calchem_hip_vxc_release", "calchem_hip_vxc_create", "calchem_hip_vxc_build",
-    "calchem_hip_vxc_build_device", "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
+    "calchem_hip_vxc_build_device", "calchem_hip_vxc_build_uks", "calchem_hip_vxc_build_uks_device",
+    "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
     "calchem_hip_grid_build",
 ]
 
@@ -82,6 +83,9 @@ def load() -> C.CDLL:
     L.calchem_hip_vxc_build.argtypes = [u64, i32, i32, dp, dp, C.POINTER(C.c_double),
                                         C.POINTER(C.c_double)]
     L.calchem_hip_vxc_build_device.argtypes = [u64, i32, i32, dp, dp, dp, vp]
+    L.calchem_hip_vxc_build_uks.argtypes = [u64, i32, i32, dp, dp, dp, dp, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
+    L.calchem_hip_vxc_build_uks_device.argtypes = [u64, i32, i32, dp, dp, dp, dp, dp, vp]
     L.calchem_hip_vxc_set_profiling.argtypes = [u64, i32]
     L.calchem_hip_vxc_stats.argtypes = [u64, C.POINTER(VxcStats)]
     L.calchem_hip_grid_build.restype = C.c_long

@@ -78,9 +78,11 @@ struct Plan {
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
     int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, n_xc_blocks = 0, reduce_waves = 1;
+    int group_max_tiles = kGroupMaxTiles;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, tiles32, tiles16, groups, group_rows,
-        group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, P32,
-        V, rho, rho_s, psub64, psub32;
+        group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
+        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b;
+    bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -93,7 +95,8 @@ struct Plan {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
                              &block_sh, &tiles32,   &tiles16,  &groups, &group_rows, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
-                             &epart,    &npart,     &en,       &P64,   &P32,       &V,        &rho, &rho_s, &psub64, &psub32};
+                             &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
+                             &rho_sb, &wv_b, &Vb_host};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
         return s;
@@ -202,6 +205,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (opt && opt->mixed_exact_threshold > 0.0) P.mixed_threshold = opt->mixed_exact_threshold;
     const int want_pts = opt ? opt->rho_tile_points : 0;
     if (want_pts != 0 && want_pts != 16 && want_pts != 32) return kBadArgs;
+    const int want_rows = opt ? opt->group_max_rows : 0;
+    if (want_rows != 0 && want_rows != 192 && want_rows != 272) return kBadArgs;
+    P.group_max_tiles = want_rows == 192 ? 12 : kGroupMaxTiles;
 
     // Device shell / AO tables.
     std::vector<DevShell> hs(n_shells);
@@ -385,7 +391,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int> hglist;
     std::vector<VPair> hpairs;
     long long scratch = 0;
-    const int max_rows = kGroupMaxTiles * 16;
+    const int max_rows = P.group_max_tiles * 16;
     auto union_rows = [&](int b, int e) {
         std::vector<int> u;
         for (int k = b; k < e; ++k) u.insert(u.end(), blist[k].begin(), blist[k].end());
@@ -488,11 +494,13 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         }
     }
 
-    if (P.n_group_list > 0 && vgroup_lds_bytes(false, P.max_group_nsh, P.max_group_nprim) > P.lds_limit) return kLimit;
+    if (P.n_group_list > 0 &&
+        vgroup_lds_bytes(false, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) > P.lds_limit)
+        return kLimit;
     P.reduce_waves = reduce_v_waves(n_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
     const size_t need = sizeof(double) * (size_t)scratch + sizeof(RedEntry) * hgg.size() +
-                        sizeof(double) * (size_t)n_ao * n_ao * 2 + sizeof(float) * (size_t)n_ao * n_ao +
+                        sizeof(double) * (size_t)n_ao * n_ao * 2 +
                         sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
     if (mem_limit > 0 && need > mem_limit) return kLimit;
 
@@ -507,7 +515,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) &&
          P.npart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) && P.en.alloc(sizeof(double) * 2) &&
-         P.P64.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.P32.alloc(sizeof(float) * (size_t)n_ao * n_ao) &&
+         P.P64.alloc(sizeof(double) * (size_t)n_ao * n_ao) &&
          P.V.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.rho.alloc(sizeof(double4) * (size_t)n_pts) &&
          P.rho_s.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.psub64.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)) &&
@@ -548,13 +556,14 @@ void resolve_profile(Plan& P) {
 }
 
 // Rho stage: gather the density blocks, then every launch class with its own
-// LDS budget.  rho goes to P.rho_s per sorted point, or to P.rho per caller
-// point when caller_order (points dropped at plan time keep what P.rho held).
-int run_rho(Plan& P, int mode, bool caller_order, const double* dP, hipStream_t s) {
+// LDS budget.  rho goes to `sorted` per sorted point, or, when sorted is
+// null, to P.rho per caller point (points dropped at plan time keep what
+// P.rho held).  With gather_mark the profiling mark after the gather is set.
+int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark) {
     GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), dP, P.n_ao, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
     if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
-    if (!caller_order) mark(P, s);
+    if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
     for (size_t c = 0; c < P.classes[0].size(); ++c) {
@@ -565,8 +574,7 @@ int run_rho(Plan& P, int mode, bool caller_order, const double* dP, hipStream_t 
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
                    g == 0 ? P.tiles32.as<RhoTile>() : P.tiles16.as<RhoTile>(), P.blocks.as<RhoBlock>(),
                    P.block_rows.as<int2>(), P.block_sh.as<int2>(), dP64, dPS,
-                   caller_order ? P.rho.as<double4>() : P.rho_s.as<double4>(),
-                   P.perm.as<int>(), caller_order ? 1 : 0};
+                   sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
         if (launch_rho(ra, mode, pts, cl, s) != hipSuccess) return kLaunchFail;
     }
     return kOk;
@@ -579,7 +587,8 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
     VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
              P.group_sh.as<int2>(), wv, P.perm.as<int>(), wv_perm, P.scratch.as<double>()};
-    if (launch_vxc_group(va, f32, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) != hipSuccess)
+    if (launch_vxc_group(va, f32, P.group_max_tiles, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) !=
+        hipSuccess)
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);
@@ -595,7 +604,7 @@ int run_build(Plan& P, int func, int prec, const double* dP, double* dV, double*
     if (!P.has_weights) return kBadArgs;
     const bool f32 = prec == CALCHEM_PREC_FP32;
     mark(P, s);
-    const int rc = run_rho(P, f32 ? kRhoMixed : kRhoFp64, false, dP, s);
+    const int rc = run_rho(P, f32 ? kRhoMixed : kRhoFp64, P.rho_s.as<double4>(), dP, s, true);
     if (rc != kOk) return rc;
     mark(P, s);
     XcArgs xa{P.rho_s.as<double4>(), P.pts.as<double4>(), P.n_pts, func, P.wv.as<double4>(), P.epart.as<double>(),
@@ -604,6 +613,45 @@ int run_build(Plan& P, int func, int prec, const double* dP, double* dV, double*
     mark(P, s);
     const int rv = run_v(P, f32, P.wv.as<double4>(), 0, dV, s, true);
     if (rv != kOk) return rv;
+    mark(P, s);
+    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
+        return kLaunchFail;
+    mark(P, s);
+    return kOk;
+}
+
+// Unrestricted build (UksVxcEngine::evaluate, vxc.rs:1662-1766): both spin
+// densities through the rho stage, the unrestricted functional, then the V
+// stage once per spin.  Profiling marks: start, after the first gather, after
+// both rho passes, after the functional, after both V passes (with their
+// reductions), an empty reduce interval, after the E/N reduction.
+int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* dPb, double* dVa, double* dVb,
+                  double* den, hipStream_t s) {
+    if (func < 0 || func > 5) return kUnsupported;
+    if (prec != CALCHEM_PREC_FP64 && prec != CALCHEM_PREC_FP32) return kUnsupported;
+    if (!P.has_weights) return kBadArgs;
+    if (!P.uks_ready) {
+        const size_t b = sizeof(double4) * (size_t)std::max(P.n_pts, 1);
+        if (!P.rho_sb.alloc(b) || !P.wv_b.alloc(b)) return kAllocFail;
+        // Points of blocks without significant AOs are never written.
+        if (hipMemset(P.rho_sb.p, 0, b) != hipSuccess) return kLaunchFail;
+        P.uks_ready = true;
+    }
+    const bool f32 = prec == CALCHEM_PREC_FP32;
+    const int mode = f32 ? kRhoMixed : kRhoFp64;
+    mark(P, s);
+    int rc = run_rho(P, mode, P.rho_s.as<double4>(), dPa, s, true);
+    if (rc == kOk) rc = run_rho(P, mode, P.rho_sb.as<double4>(), dPb, s, false);
+    if (rc != kOk) return rc;
+    mark(P, s);
+    XcUksArgs xa{P.rho_s.as<double4>(), P.rho_sb.as<double4>(), P.pts.as<double4>(), P.n_pts, func,
+                 P.wv.as<double4>(), P.wv_b.as<double4>(), P.epart.as<double>(), P.npart.as<double>()};
+    if (launch_xc_uks(xa, s) != hipSuccess) return kLaunchFail;
+    mark(P, s);
+    rc = run_v(P, f32, P.wv.as<double4>(), 0, dVa, s, false);
+    if (rc == kOk) rc = run_v(P, f32, P.wv_b.as<double4>(), 0, dVb, s, false);
+    if (rc != kOk) return rc;
+    mark(P, s);
     mark(P, s);
     if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
         return kLaunchFail;
@@ -671,7 +719,7 @@ int calchem_hip_vxc_eval_rho(uint64_t grid_id, const double* P_mat, double* rho_
     hipStream_t s = P.stream;
     if (hipMemcpyAsync(P.P64.p, P_mat, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess) return kLaunchFail;
     if (launch_fill_zero(P.rho.as<double>(), 4LL * n_pts, s) != hipSuccess) return kLaunchFail;
-    const int rc = run_rho(P, kRhoFp64, true, P.P64.as<double>(), s);
+    const int rc = run_rho(P, kRhoFp64, nullptr, P.P64.as<double>(), s, false);
     if (rc != kOk) return rc;
     if (hipMemcpyAsync(rho_grad, P.rho.p, sizeof(double4) * (size_t)n_pts, hipMemcpyDeviceToHost, s) != hipSuccess)
         return kLaunchFail;
@@ -753,6 +801,51 @@ int calchem_hip_vxc_build_device(uint64_t handle, int xc, int precision, const d
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     return run_build(P, xc, precision, d_P, d_V, d_exc_nelec, static_cast<hipStream_t>(stream));
+}
+
+int calchem_hip_vxc_build_uks(uint64_t handle, int xc, int precision, const double* Pa, const double* Pb,
+                              double* Va_out, double* Vb_out, double* exc, double* nelec) {
+    if (!Pa || !Pb || !Va_out || !Vb_out || !exc || handle == 0) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    hipStream_t s = P.stream;
+    const size_t nn = (size_t)P.n_ao * P.n_ao;
+    // The host-buffer form stages the beta density and V in buffers of its
+    // own, allocated on its first call.
+    if (!P.Pb_host.p && (!P.Pb_host.alloc(sizeof(double) * nn) || !P.Vb_host.alloc(sizeof(double) * nn)))
+        return kAllocFail;
+    double* dPb = P.Pb_host.as<double>();
+    if (hipMemcpyAsync(P.P64.p, Pa, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(dPb, Pb, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess)
+        return kLaunchFail;
+    const int rc = run_build_uks(P, xc, precision, P.P64.as<double>(), dPb, P.V.as<double>(),
+                                 P.Vb_host.as<double>(), P.en.as<double>(), s);
+    if (rc != kOk) return rc;
+    double en[2] = {0.0, 0.0};
+    if (hipMemcpyAsync(Va_out, P.V.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(Vb_out, P.Vb_host.p, sizeof(double) * nn, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(en, P.en.p, sizeof(double) * 2, hipMemcpyDeviceToHost, s) != hipSuccess)
+        return kLaunchFail;
+    if (hipStreamSynchronize(s) != hipSuccess) return kLaunchFail;
+    *exc = en[0];
+    if (nelec) *nelec = en[1];
+    return kOk;
+}
+
+int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, const double* d_Pa, const double* d_Pb,
+                                     double* d_Va, double* d_Vb, double* d_exc_nelec, void* stream) {
+    if (!d_Pa || !d_Pb || !d_Va || !d_Vb || !d_exc_nelec || handle == 0) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    return run_build_uks(P, xc, precision, d_Pa, d_Pb, d_Va, d_Vb, d_exc_nelec, static_cast<hipStream_t>(stream));
 }
 
 int calchem_hip_vxc_set_profiling(uint64_t handle, int on) {

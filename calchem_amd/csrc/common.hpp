@@ -86,7 +86,7 @@ struct VPair {
 constexpr int kVTile = 64;       // tile of the tile-pair V kernel
 constexpr int kVPts = 16;        // points staged per V-stage step
 constexpr int kThreads = 256;    // 4 wavefronts of 64
-constexpr int kGroupMaxTiles = 17;   // 16-row tiles per group it accepts (272 rows)
+constexpr int kGroupMaxTiles = 17;   // default 16-row tiles per register-resident V group (272 rows)
 // Elements after the last gathered density block: the rho kernel's look-ahead
 // loads may read up to 3 chunks of 16 past a row's end (values unused).
 constexpr int kPsubTail = 64;

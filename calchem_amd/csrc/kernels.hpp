@@ -30,6 +30,19 @@ struct XcArgs {
     double* n_part;
 };
 
+// Unrestricted functional stage: both spin densities in, both wv out.
+struct XcUksArgs {
+    const double4* rho_a;   // per sorted point (rho_a, grad rho_a)
+    const double4* rho_b;
+    const double4* pts;
+    int n_pts;
+    int func;
+    double4* wv_a;          // (w v_a / 2, w u_a), u_a = 2 v_gaa grad rho_a + v_gab grad rho_b
+    double4* wv_b;
+    double* e_part;
+    double* n_part;
+};
+
 // Launch class of rho tiles: blocks of similar size share one LDS budget.
 struct RhoClass {
     int tile_begin, n_tiles;  // range in the tile list of one point granularity
@@ -100,9 +113,13 @@ enum RhoMode { kRhoFp64 = 0, kRhoMixed = 1 };
 size_t rho_lds_bytes(int mode, int pts, const RhoClass& c);
 hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hipStream_t s);
 hipError_t launch_xc(const XcArgs& a, hipStream_t s);
+hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);
-size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim);
-hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh, int max_prim, hipStream_t s);
+// Register-resident V kernel over groups of at most max_tiles x 16 rows
+// (max_tiles = 17 or 12).
+size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim);
+hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
+                            hipStream_t s);
 int reduce_v_waves(int n_ao, size_t lds_limit);
 hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s);
 hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s);

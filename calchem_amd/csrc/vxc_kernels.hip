@@ -338,6 +338,52 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
     }
 }
 
+// Unrestricted functional stage, one thread per sorted point
+// (crates/calchem-dft/src/vxc.rs:1141-1348, accumulate_uks_points): spin
+// densities clamped at zero, skipped when rho_a + rho_b < 1e-18; gamma_aa,
+// gamma_ab, gamma_bb from the spin gradients; u_a = 2 v_gaa grad rho_a +
+// v_gab grad rho_b (and a <-> b); wv_s = (w v_s / 2, w u_s); N sums the
+// unclamped densities.
+__global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
+    __shared__ double se[kThreads], sn[kThreads];
+    const int g = blockIdx.x * kThreads + threadIdx.x;
+    double e = 0.0, ne = 0.0;
+    if (g < a.n_pts) {
+        const double4 ra = a.rho_a[g], rb = a.rho_b[g];
+        const double w = a.pts[g].w;
+        ne = w * (ra.x + rb.x);
+        double4 oa = make_double4(0.0, 0.0, 0.0, 0.0), ob = oa;
+        const double pa = ra.x > 0.0 ? ra.x : 0.0, pb = rb.x > 0.0 ? rb.x : 0.0;
+        if (pa + pb >= 1e-18) {  // DENSITY_SCREEN, vxc.rs:23
+            const double gaa = ra.y * ra.y + ra.z * ra.z + ra.w * ra.w;
+            const double gab = ra.y * rb.y + ra.z * rb.z + ra.w * rb.w;
+            const double gbb = rb.y * rb.y + rb.z * rb.z + rb.w * rb.w;
+            const XcOut o = eval_xc(a.func, pa, pb, gaa, gab, gbb);
+            e = w * o.f;
+            oa = make_double4(0.5 * w * o.va, w * (2.0 * o.gaa * ra.y + o.gab * rb.y),
+                              w * (2.0 * o.gaa * ra.z + o.gab * rb.z), w * (2.0 * o.gaa * ra.w + o.gab * rb.w));
+            ob = make_double4(0.5 * w * o.vb, w * (2.0 * o.gbb * rb.y + o.gab * ra.y),
+                              w * (2.0 * o.gbb * rb.z + o.gab * ra.z), w * (2.0 * o.gbb * rb.w + o.gab * ra.w));
+        }
+        a.wv_a[g] = oa;
+        a.wv_b[g] = ob;
+    }
+    se[threadIdx.x] = e;
+    sn[threadIdx.x] = ne;
+    __syncthreads();
+    for (int s = kThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) {
+            se[threadIdx.x] += se[threadIdx.x + s];
+            sn[threadIdx.x] += sn[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        a.e_part[blockIdx.x] = se[0];
+        a.n_part[blockIdx.x] = sn[0];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // V stage
 // ---------------------------------------------------------------------------
@@ -444,38 +490,47 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
 // acc += Z_I Phi_J^T + Phi_I Z_J^T.  FP64: 8 waves, AO evaluation in FP64.
 // FP32 (mixed build): 16 waves (half the pairs per wave, four waves per SIMD)
 // and AO evaluation in FP32.
-template <typename T>
+template <typename T, int MAXT>
 struct VGroupCfg;
 template <>
-struct VGroupCfg<double> {
-    static constexpr int kWaves = 8;
+struct VGroupCfg<double, 17> {
+    static constexpr int kWaves = 8;  // 20 FP64 pairs per wave: two waves per SIMD
     using Rad = double2;
 };
 template <>
-struct VGroupCfg<float> {
+struct VGroupCfg<double, 12> {
+    static constexpr int kWaves = 16;  // 5 FP64 pairs per wave: four waves per SIMD
+    using Rad = double2;
+};
+template <int MAXT>
+struct VGroupCfg<float, MAXT> {
     static constexpr int kWaves = 16;
     using Rad = float2;
 };
 
-template <typename T>
+template <int MAXT>
+__host__ __device__ constexpr int vgroup_max_pairs() {
+    return MAXT * (MAXT + 1) / 2;
+}
+
+template <typename T, int MAXT>
 __host__ __device__ constexpr int vgroup_pairs_per_wave() {
-    return (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + VGroupCfg<T>::kWaves - 1) / VGroupCfg<T>::kWaves;
+    return (vgroup_max_pairs<MAXT>() + VGroupCfg<T, MAXT>::kWaves - 1) / VGroupCfg<T, MAXT>::kWaves;
 }
 
-// Pair table entries, rounded up to keep 16-byte alignment after it.
-constexpr int kGroupMaxPairsPad = (kGroupMaxTiles * (kGroupMaxTiles + 1) / 2 + 1) & ~1;
-
-template <typename T>
+// LDS before the per-shell tables; the pair table is rounded up to keep
+// 16-byte alignment after it.
+template <typename T, int MAXT>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
-    return sizeof(double4) * 2 * kVPts + sizeof(int2) * kGroupMaxPairsPad + sizeof(RowL) * kGroupMaxTiles * 16 +
-           sizeof(T) * 2 * kGroupMaxTiles * 16 * (kVPts + 1);
+    return sizeof(double4) * 2 * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+           sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * (kVPts + 1);
 }
 
-template <typename T>
-__global__ __launch_bounds__(VGroupCfg<T>::kWaves * 64) void k_vxc_group(VArgs a) {
+template <typename T, int MAXT>
+__global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group(VArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int NW = VGroupCfg<T>::kWaves, NT = NW * 64, PPW = vgroup_pairs_per_wave<T>();
-    using Rad = typename VGroupCfg<T>::Rad;
+    constexpr int NW = VGroupCfg<T, MAXT>::kWaves, NT = NW * 64, PPW = vgroup_pairs_per_wave<T, MAXT>();
+    using Rad = typename VGroupCfg<T, MAXT>::Rad;
     constexpr int LD = kVPts + 1;
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh;
@@ -487,13 +542,13 @@ __global__ __launch_bounds__(VGroupCfg<T>::kWaves * 64) void k_vxc_group(VArgs a
     double4* sWv = reinterpret_cast<double4*>(q);
     q += sizeof(double4) * kVPts;
     int2* sPair = reinterpret_cast<int2*>(q);
-    q += sizeof(int2) * kGroupMaxPairsPad;
+    q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
-    q += sizeof(RowL) * kGroupMaxTiles * 16;
+    q += sizeof(RowL) * MAXT * 16;
     T* sP = reinterpret_cast<T*>(q);
-    q += sizeof(T) * kGroupMaxTiles * 16 * LD;
+    q += sizeof(T) * MAXT * 16 * LD;
     T* sZ = reinterpret_cast<T*>(q);
-    q += sizeof(T) * kGroupMaxTiles * 16 * LD;
+    q += sizeof(T) * MAXT * 16 * LD;
     Rad* sRad = reinterpret_cast<Rad*>(q);  // [nsh][kVPts]
     q += sizeof(Rad) * ((nsh * kVPts + 1) & ~1);
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
@@ -743,6 +798,12 @@ hipError_t launch_xc(const XcArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s) {
+    if (a.n_pts <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xc_uks, dim3((a.n_pts + kThreads - 1) / kThreads), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
     if (n_pairs <= 0) return hipSuccess;
     if (fp32)
@@ -752,14 +813,20 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
     return hipGetLastError();
 }
 
-size_t vgroup_lds_bytes(bool fp32, int max_nsh, int max_prim) {
-    const size_t fixed = fp32 ? vgroup_lds_fixed<float>() : vgroup_lds_fixed<double>();
-    const size_t rad = fp32 ? sizeof(float2) * (size_t)((max_nsh * kVPts + 1) & ~1)
-                            : sizeof(double2) * (size_t)max_nsh * kVPts;
-    return fixed + rad + sizeof(DevShell) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
+template <typename T, int MAXT>
+static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim) {
+    const size_t rad = sizeof(typename VGroupCfg<T, MAXT>::Rad) * (size_t)((max_nsh * kVPts + 1) & ~1);
+    return vgroup_lds_fixed<T, MAXT>() + rad + sizeof(DevShell) * (size_t)max_nsh +
+           sizeof(double2) * (size_t)max_prim;
 }
 
-template <typename T>
+size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim) {
+    if (max_tiles == 12)
+        return fp32 ? vgroup_lds_bytes_t<float, 12>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 12>(max_nsh, max_prim);
+    return fp32 ? vgroup_lds_bytes_t<float, 17>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 17>(max_nsh, max_prim);
+}
+
+template <typename T, int MAXT>
 static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, int max_prim, hipStream_t s) {
     static int attr_set = 0;
     if (!attr_set) {
@@ -767,20 +834,25 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vxc_group<T>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vxc_group<T, MAXT>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
         attr_set = 1;
     }
     if (n_groups <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_vxc_group<T>, dim3(n_groups), dim3(VGroupCfg<T>::kWaves * 64),
-                       vgroup_lds_bytes(sizeof(T) == 4, max_nsh, max_prim), s, a);
+    constexpr int threads = VGroupCfg<T, MAXT>::kWaves * 64;
+    const size_t lds = vgroup_lds_bytes_t<T, MAXT>(max_nsh, max_prim);
+    hipLaunchKernelGGL((k_vxc_group<T, MAXT>), dim3(n_groups), dim3(threads), lds, s, a);
     return hipGetLastError();
 }
 
-hipError_t launch_vxc_group(const VArgs& a, bool fp32, int n_groups, int max_nsh, int max_prim, hipStream_t s) {
-    return fp32 ? launch_vxc_group_t<float>(a, n_groups, max_nsh, max_prim, s)
-                : launch_vxc_group_t<double>(a, n_groups, max_nsh, max_prim, s);
+hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
+                            hipStream_t s) {
+    if (max_tiles == 12)
+        return fp32 ? launch_vxc_group_t<float, 12>(a, n_groups, max_nsh, max_prim, s)
+                    : launch_vxc_group_t<double, 12>(a, n_groups, max_nsh, max_prim, s);
+    return fp32 ? launch_vxc_group_t<float, 17>(a, n_groups, max_nsh, max_prim, s)
+                : launch_vxc_group_t<double, 17>(a, n_groups, max_nsh, max_prim, s);
 }
 
 // Waves (row copies) per reduction workgroup that fit in LDS: up to 16.

@@ -4,6 +4,7 @@ RksVxcEngine(basis, grid)            RksVxcEngine::new_with_gpu (vxc.rs:429-716)
 engine.evaluate(basis, grid, xc, P)  RksVxcEngine::evaluate (vxc.rs:720-985) -> (E_xc, V)
 engine.evaluate_full(...)            same, plus N = sum_g w_g rho_g
 engine.telemetry()                   RksVxcEngine::telemetry (vxc.rs:420-425)
+UksVxcEngine                         UksVxcEngine (vxc.rs:1473-1766): (E_xc, Va, Vb)
 GpuVxcGrid                           calchem_cuda::GpuVxcGrid (lib.rs:1178-1306): the
                                      reference GPU backend contract (rho on the device,
                                      functional on the host, V on the device).
@@ -40,7 +41,7 @@ def _records(basis: BasisSet):
 class RksVxcEngine:
     def __init__(self, basis: BasisSet, grid: MolecularGrid, block_points: int = 0,
                  group_points: int = 0, memory_limit_bytes: int = 0, mixed_exact_threshold: float = 0.0,
-                 rho_tile_points: int = 0):
+                 rho_tile_points: int = 0, group_max_rows: int = 0):
         self._lib = _native.require_device()
         self.n_ao = basis.n_ao
         self.n_points = grid.n_points
@@ -49,7 +50,7 @@ class RksVxcEngine:
         shells, prims, comps = _records(basis)
         xyzw = grid.xyzw()
         opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold,
-                                  rho_tile_points)
+                                  rho_tile_points, group_max_rows)
         h = _native.new_handle()
         rc = self._lib.calchem_hip_vxc_create(
             _native.ptr(shells), len(shells), _native.ptr(prims), len(prims), _native.ptr(comps),
@@ -113,6 +114,56 @@ class RksVxcEngine:
     def telemetry(self) -> Tuple[int, int]:
         """(GPU kernel invocations, CPU fallback invocations); the latter is always 0."""
         return self._kernel_invocations, 0
+
+
+class UksVxcEngine(RksVxcEngine):
+    """Unrestricted engine, mirroring ``calchem_dft::vxc::UksVxcEngine``.
+
+    UksVxcEngine(basis, grid)                  UksVxcEngine::new_with_gpu (vxc.rs:1473-1660)
+    engine.evaluate(basis, grid, xc, Pa, Pb)   UksVxcEngine::evaluate (vxc.rs:1662-1766) -> (E_xc, Va, Vb)
+    engine.evaluate_full(xc, Pa, Pb)           same, plus N = sum_g w_g (rho_a + rho_b)
+
+    It shares the plan machinery of the restricted engine (one plan per
+    geometry); the restricted entry points stay usable on the same object.
+    """
+
+    @classmethod
+    def new_with_gpu(cls, basis: BasisSet, grid: MolecularGrid, use_gpu: bool = True) -> "UksVxcEngine":
+        if not use_gpu:
+            raise ValueError("calchem_amd has no CPU Vxc engine; use_gpu must be True")
+        return cls(basis, grid)
+
+    def evaluate_full(self, func: str, Pa: np.ndarray, Pb: np.ndarray = None,
+                      precision: str = "fp64") -> Tuple[float, np.ndarray, np.ndarray, float]:
+        if Pb is None:
+            raise TypeError("UksVxcEngine.evaluate_full needs both spin densities")
+        n = self.n_ao
+        Pa = np.ascontiguousarray(Pa, dtype=np.float64)
+        Pb = np.ascontiguousarray(Pb, dtype=np.float64)
+        for P in (Pa, Pb):
+            if P.shape != (n, n):
+                raise ValueError(f"density has shape {P.shape}, expected {(n, n)}")
+        Va, Vb = np.empty((n, n)), np.empty((n, n))
+        e, ne = C.c_double(), C.c_double()
+        rc = self._lib.calchem_hip_vxc_build_uks(self.handle, xc_id(func), PRECISIONS[precision], _native.ptr(Pa),
+                                                 _native.ptr(Pb), _native.ptr(Va), _native.ptr(Vb), C.byref(e),
+                                                 C.byref(ne))
+        _native.check(rc, "calchem_hip_vxc_build_uks")
+        self._kernel_invocations += 2
+        return e.value, Va, Vb, ne.value
+
+    def evaluate(self, basis: Optional[BasisSet], grid: Optional[MolecularGrid], func: str, Pa: np.ndarray,
+                 Pb: np.ndarray = None, precision: str = "fp64") -> Tuple[float, np.ndarray, np.ndarray]:
+        e, Va, Vb, _ = self.evaluate_full(func, Pa, Pb, precision)
+        return e, Va, Vb
+
+    def build_device_uks(self, func: str, d_Pa: int, d_Pb: int, d_Va: int, d_Vb: int, d_exc_nelec: int,
+                         stream: int = 0, precision: str = "fp64") -> None:
+        """Enqueue an unrestricted build on device pointers (no synchronisation)."""
+        rc = self._lib.calchem_hip_vxc_build_uks_device(self.handle, xc_id(func), PRECISIONS[precision], d_Pa,
+                                                        d_Pb, d_Va, d_Vb, d_exc_nelec, stream or None)
+        _native.check(rc, "calchem_hip_vxc_build_uks_device")
+        self._kernel_invocations += 2
 
 
 class GpuVxcGrid:

@@ -79,7 +79,8 @@ typedef struct {
     size_t memory_limit_bytes;     /* 0 = no limit */
     double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.25) */
     int32_t rho_tile_points;       /* points per rho workgroup: 0 = default (16), 16 or 32 */
-    int32_t reserved[3];
+    int32_t group_max_rows;        /* AO rows per register-resident V group: 0 = default (272), 192 or 272 */
+    int32_t reserved[2];
 } CalchemVxcOptions;
 
 typedef struct {
@@ -132,6 +133,19 @@ int calchem_hip_vxc_accum(uint64_t grid_id, const double *wv, double *V_out, int
 /* Replaces calchem_cuda_vxc_release (cuda_kernels.cu:7547-7562; lib.rs:334,
  * GpuVxcGrid::drop lib.rs:1184-1193).  Frees handles of either kind. */
 int calchem_hip_vxc_release(uint64_t grid_id);
+
+/* ---- Unrestricted build ----
+ * Replaces UksVxcEngine::evaluate (crates/calchem-dft/src/vxc.rs:1662-1766)
+ * and evaluate_vxc_uks (vxc.rs:1352-1470): spin densities Pa, Pb (n_ao^2,
+ * row-major) in; Va, Vb (full symmetric), E_xc and N = sum_g w_g (rho_a + rho_b)
+ * out.  Same plan as the restricted build; the first call allocates the
+ * second spin's per-point buffers.  _device: device pointers, asynchronous
+ * on `stream`, d_exc_nelec[2] = (E_xc, N). */
+int calchem_hip_vxc_build_uks(uint64_t handle, int xc, int precision, const double *Pa, const double *Pb,
+                              double *Va_out, double *Vb_out, double *exc, double *nelec);
+int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, const double *d_Pa,
+                                     const double *d_Pb, double *d_Va, double *d_Vb, double *d_exc_nelec,
+                                     void *stream);
 
 /* ---- Fused build (functional on the device) ----
  * Replaces the per-geometry setup of RksVxcEngine::new_with_gpu
