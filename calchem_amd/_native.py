@@ -54,6 +54,8 @@ EXPORTS = [
     "calchem_hip_vxc_upload_grid", "calchem_hip_vxc_eval_rho", "calchem_hip_vxc_accum",
     "calchem_hip_vxc_release", "calchem_hip_vxc_create", "calchem_hip_vxc_build",
     "calchem_hip_vxc_build_device", "calchem_hip_vxc_build_uks", "calchem_hip_vxc_build_uks_device",
+    "calchem_hip_vxc_create_batched", "calchem_hip_vxc_build_batched", "calchem_hip_vxc_build_batched_device",
+    "calchem_hip_vxc_graph_create", "calchem_hip_vxc_graph_launch", "calchem_hip_vxc_graph_destroy",
     "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
     "calchem_hip_grid_build",
 ]
@@ -86,6 +88,12 @@ def load() -> C.CDLL:
     L.calchem_hip_vxc_build_uks.argtypes = [u64, i32, i32, dp, dp, dp, dp, C.POINTER(C.c_double),
                                             C.POINTER(C.c_double)]
     L.calchem_hip_vxc_build_uks_device.argtypes = [u64, i32, i32, dp, dp, dp, dp, dp, vp]
+    L.calchem_hip_vxc_create_batched.argtypes = [i32, vp, vp, vp, vp, vp, i32, vp, i32, dp, vp, u64]
+    L.calchem_hip_vxc_build_batched.argtypes = [u64, i32, i32, dp, dp, dp, dp]
+    L.calchem_hip_vxc_build_batched_device.argtypes = [u64, i32, i32, dp, dp, dp, vp]
+    L.calchem_hip_vxc_graph_create.argtypes = [u64, i32, i32, dp, dp, dp, C.POINTER(C.c_uint64)]
+    L.calchem_hip_vxc_graph_launch.argtypes = [u64, u64, vp]
+    L.calchem_hip_vxc_graph_destroy.argtypes = [u64, u64]
     L.calchem_hip_vxc_set_profiling.argtypes = [u64, i32]
     L.calchem_hip_vxc_stats.argtypes = [u64, C.POINTER(VxcStats)]
     L.calchem_hip_grid_build.restype = C.c_long

@@ -142,6 +142,14 @@ def batching_cluster_sizes(n_clusters: int = 64, seed: int = 3, lo: int = 2, hi:
     return [lo + int(rng.uniform() * (hi - lo + 1)) for _ in range(n_clusters)]
 
 
+def batching_clusters(n_clusters: int = 64, seed: int = 3) -> List[Molecule]:
+    """The many-molecule workload: clusters of uneven sizes (2-32 waters, drawn
+    with `seed`), cluster k placed by the water-cluster rule with seed
+    1000 * seed + k."""
+    sizes = batching_cluster_sizes(n_clusters, seed)
+    return [water_cluster(n, 1000 * seed + k) for k, n in enumerate(sizes)]
+
+
 def random_density(basis: BasisSet, n_occ: int, seed: int, S: np.ndarray = None) -> np.ndarray:
     n = basis.n_ao
     if n_occ <= 0 or n_occ > n:

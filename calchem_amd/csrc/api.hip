@@ -77,17 +77,23 @@ struct Plan {
     double flops_rho = 0, flops_v = 0, flops_rho_pad[2] = {0, 0}, flops_v_pad = 0, flops_rho_fp64_rows = 0;
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
-    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, n_xc_blocks = 0, reduce_waves = 1;
+    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, reduce_waves = 1;
     int group_max_tiles = kGroupMaxTiles;
+    int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
+    long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
+    std::vector<MolInfo> mol_host;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, tiles32, tiles16, groups, group_rows,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
-        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b;
+        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg;
     bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
     Profile prof;
+    std::vector<hipGraphExec_t> graphs;  // captured builds; id = index + 1, null once destroyed
     ~Plan() {
+        for (hipGraphExec_t g : graphs)
+            if (g) hipGraphExecDestroy(g);
         for (hipEvent_t e : prof.ev) hipEventDestroy(e);
         if (stream) hipStreamDestroy(stream);
     }
@@ -96,7 +102,7 @@ struct Plan {
                              &block_sh, &tiles32,   &tiles16,  &groups, &group_rows, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
-                             &rho_sb, &wv_b, &Vb_host};
+                             &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
         return s;
@@ -163,34 +169,65 @@ bool upload(DevBuf& d, const std::vector<T>& h) {
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// One molecule of a plan as the caller hands it in: its shells
+// [shell_begin, shell_end) (ao_offset local to the molecule), its points
+// [pt_begin, pt_end) of the caller's point array, its AO count.
+struct MolSpec {
+    int shell_begin, shell_end, pt_begin, pt_end, n_ao;
+};
+
 int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr, int n_prims,
-                const CalchemAoComp* cp, int n_comps, const double* xyzw, int stride, int n_pts, int n_ao,
-                bool has_w, const CalchemVxcOptions* opt, uint64_t id) {
-    if (!sh || !pr || !cp || !xyzw || n_shells <= 0 || n_prims <= 0 || n_comps <= 0 || n_pts <= 0 || n_ao <= 0 ||
-        id == 0)
+                const CalchemAoComp* cp, int n_comps, const double* xyzw, int stride, int n_pts,
+                const std::vector<MolSpec>& mspec, bool has_w, const CalchemVxcOptions* opt, uint64_t id) {
+    if (!sh || !pr || !cp || !xyzw || n_shells <= 0 || n_prims <= 0 || n_comps <= 0 || n_pts <= 0 ||
+        mspec.empty() || id == 0)
         return kBadArgs;
     if (!device_available()) return kNoDevice;
+    const int n_mol = static_cast<int>(mspec.size());
+    long long n_ao_ll = 0;
+    for (const MolSpec& m : mspec) {
+        if (m.n_ao <= 0 || m.shell_begin < 0 || m.shell_end > n_shells || m.shell_begin >= m.shell_end ||
+            m.pt_begin < 0 || m.pt_end > n_pts || m.pt_begin > m.pt_end)
+            return kBadArgs;
+        n_ao_ll += m.n_ao;
+    }
+    for (int k = 0; k + 1 < n_mol; ++k)
+        if (mspec[k].shell_end != mspec[k + 1].shell_begin || mspec[k].pt_end != mspec[k + 1].pt_begin)
+            return kMismatch;
+    if (mspec[0].shell_begin != 0 || mspec[n_mol - 1].shell_end != n_shells || mspec[0].pt_begin != 0 ||
+        mspec[n_mol - 1].pt_end != n_pts || n_ao_ll > (1 << 30))
+        return kMismatch;
+    const int n_ao = static_cast<int>(n_ao_ll);
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
         auto it = g_table.find(id);
         if (it != g_table.end())
-            return (it->second->n_in == n_pts && it->second->n_ao == n_ao) ? kOk : kMismatch;
+            return (it->second->n_in == n_pts && it->second->n_ao == n_ao && it->second->n_mol == n_mol)
+                       ? kOk
+                       : kMismatch;
     }
-    // Validate shell records against the arrays they index.
-    int ao_count = 0;
-    for (int s = 0; s < n_shells; ++s) {
-        const CalchemAoShell& x = sh[s];
-        const int nf = (x.n_funcs);
-        if (x.prim_start < 0 || x.n_prims <= 0 || x.prim_start + x.n_prims > n_prims || x.comp_start < 0 ||
-            nf <= 0 || x.comp_start + nf > n_comps || x.ao_offset < 0 || x.ao_offset + nf > n_ao)
-            return kMismatch;
-        for (int c = 0; c < nf; ++c) {
-            const CalchemAoComp& q = cp[x.comp_start + c];
-            if (q.lx < 0 || q.ly < 0 || q.lz < 0 || q.lx > 3 || q.ly > 3 || q.lz > 3) return kUnsupported;
+    // Validate shell records against the arrays they index; AO offsets are
+    // local to each molecule.
+    std::vector<int> shell_mol(n_shells), ao_base(n_mol);
+    for (int k = 0, base = 0; k < n_mol; ++k) {
+        ao_base[k] = base;
+        int ao_count = 0;
+        for (int s = mspec[k].shell_begin; s < mspec[k].shell_end; ++s) {
+            const CalchemAoShell& x = sh[s];
+            const int nf = (x.n_funcs);
+            if (x.prim_start < 0 || x.n_prims <= 0 || x.prim_start + x.n_prims > n_prims || x.comp_start < 0 ||
+                nf <= 0 || x.comp_start + nf > n_comps || x.ao_offset < 0 || x.ao_offset + nf > mspec[k].n_ao)
+                return kMismatch;
+            for (int c = 0; c < nf; ++c) {
+                const CalchemAoComp& q = cp[x.comp_start + c];
+                if (q.lx < 0 || q.ly < 0 || q.lz < 0 || q.lx > 3 || q.ly > 3 || q.lz > 3) return kUnsupported;
+            }
+            ao_count += nf;
+            shell_mol[s] = k;
         }
-        ao_count += nf;
+        if (ao_count != mspec[k].n_ao) return kMismatch;
+        base += mspec[k].n_ao;
     }
-    if (ao_count != n_ao) return kMismatch;
 
     auto plan = std::make_shared<Plan>();
     Plan& P = *plan;
@@ -199,6 +236,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     P.n_shells = n_shells;
     P.n_in = n_pts;
     P.has_weights = has_w;
+    P.n_mol = n_mol;
     const int nb_rho = (opt && opt->block_points > 0) ? round_up(opt->block_points, 32) : 256;
     const int nb_v = (opt && opt->group_points > 0) ? opt->group_points : 1024;
     const size_t mem_limit = opt ? opt->memory_limit_bytes : 0;
@@ -218,10 +256,11 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (int s = 0; s < n_shells; ++s) {
         const CalchemAoShell& x = sh[s];
         int L = 0;
+        const int gao = ao_base[shell_mol[s]] + x.ao_offset;
         for (int c = 0; c < x.n_funcs; ++c) {
             const CalchemAoComp& q = cp[x.comp_start + c];
             L = std::max(L, q.lx + q.ly + q.lz);
-            DevAo& a = hao[x.ao_offset + c];
+            DevAo& a = hao[gao + c];
             a.scale = q.scale;
             a.shell = s;
             a.lx = q.lx;
@@ -229,7 +268,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             a.lz = q.lz;
         }
         lmax = std::max(lmax, L);
-        hs[s] = DevShell{x.center[0], x.center[1], x.center[2], x.r2_cutoff, x.prim_start, x.n_prims, L, x.ao_offset};
+        hs[s] = DevShell{x.center[0], x.center[1], x.center[2], x.r2_cutoff, x.prim_start, x.n_prims, L, gao};
     }
 
     // Points: keep |w| >= 1e-24 (vxc.rs:454,539), sort into compact blocks.
@@ -244,11 +283,43 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         if (std::fabs(w) >= 1e-24) keep.push_back(i);
     }
     P.n_pts = static_cast<int>(keep.size());
+    // Blocks never mix molecules: each molecule's kept points are blocked on
+    // their own (keep is in caller order, so a molecule's points are a range).
     std::vector<std::pair<int, int>> leaves;
-    if (P.n_pts > 0) kd_blocks(keep, 0, P.n_pts, all, nb_rho, leaves);
+    std::vector<int> block_mol;
+    P.mol_host.assign(n_mol, MolInfo{});
+    for (int k = 0, kb = 0; k < n_mol; ++k) {
+        int ke = kb;
+        while (ke < P.n_pts && keep[ke] < mspec[k].pt_end) ++ke;
+        const size_t l0 = leaves.size();
+        if (ke > kb) kd_blocks(keep, kb, ke, all, nb_rho, leaves);
+        block_mol.resize(leaves.size(), k);
+        MolInfo& mi = P.mol_host[k];
+        mi.ao_base = ao_base[k];
+        mi.n_ao = mspec[k].n_ao;
+        mi.p_off = P.p_elems;
+        P.p_elems += (long long)mi.n_ao * mi.n_ao;
+        mi.pt_begin = kb;
+        mi.n_pts = ke - kb;
+        P.max_mol_ao = std::max(P.max_mol_ao, mi.n_ao);
+        (void)l0;
+        kb = ke;
+    }
     std::vector<double4> spts(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) spts[i] = all[keep[i]];
     P.n_blocks = static_cast<int>(leaves.size());
+    // Functional-stage workgroups: 256 points of one molecule each.
+    std::vector<int2> hxc;
+    std::vector<int> hao_mol(n_ao);
+    for (int k = 0; k < n_mol; ++k) {
+        MolInfo& mi = P.mol_host[k];
+        mi.xc_wg_begin = static_cast<int>(hxc.size());
+        for (int o = 0; o < mi.n_pts; o += kThreads)
+            hxc.push_back(make_int2(mi.pt_begin + o, std::min(kThreads, mi.n_pts - o)));
+        mi.n_xc_wg = static_cast<int>(hxc.size()) - mi.xc_wg_begin;
+        for (int i = 0; i < mi.n_ao; ++i) hao_mol[mi.ao_base + i] = k;
+    }
+    P.n_xc_wg = static_cast<int>(hxc.size());
 
     // Candidate shells per block: |R_shell - c| <= r_box + r_cut with c, r_box
     // the block's bounding-box centre and half diagonal (vxc.rs:446-493).
@@ -270,12 +341,13 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         const double rb = std::sqrt((hi[0] - cx) * (hi[0] - cx) + (hi[1] - cy) * (hi[1] - cy) +
                                     (hi[2] - cz) * (hi[2] - cz));
         cand_begin[b] = static_cast<int>(cand_ao.size());
-        for (int s = 0; s < n_shells; ++s) {
+        const MolSpec& ms = mspec[block_mol[b]];
+        for (int s = ms.shell_begin; s < ms.shell_end; ++s) {
             const double dx = hs[s].cx - cx, dy = hs[s].cy - cy, dz = hs[s].cz - cz;
             const double th = rb + rcut[s];
             if (dx * dx + dy * dy + dz * dz <= th * th) {
                 for (int c = 0; c < sh[s].n_funcs; ++c) {
-                    cand_ao.push_back(sh[s].ao_offset + c);
+                    cand_ao.push_back(hs[s].ao_offset + c);
                     cand_rng.push_back(make_int2(leaves[b].first, leaves[b].second - leaves[b].first));
                 }
             }
@@ -324,6 +396,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         std::vector<int> rows = big;
         rows.insert(rows.end(), small.begin(), small.end());
         RhoBlock B{};
+        B.mol = block_mol[b];
         B.ao_begin = static_cast<int>(hrows.size());
         B.m = m;
         B.m_pad = mp;
@@ -402,7 +475,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (int b = 0; b < P.n_blocks;) {
         int e = b + 1, npt = leaves[b].second - leaves[b].first;
         std::vector<int> u = union_rows(b, e);
-        while (e < P.n_blocks && npt + (leaves[e].second - leaves[e].first) <= nb_v) {
+        while (e < P.n_blocks && block_mol[e] == block_mol[b] && npt + (leaves[e].second - leaves[e].first) <= nb_v) {
             std::vector<int> u2 = union_rows(b, e + 1);
             if ((int)u2.size() > max_rows && (int)u.size() <= max_rows) break;
             u.swap(u2);
@@ -497,14 +570,14 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (P.n_group_list > 0 &&
         vgroup_lds_bytes(false, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) > P.lds_limit)
         return kLimit;
-    P.reduce_waves = reduce_v_waves(n_ao, P.lds_limit);
+    P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
     const size_t need = sizeof(double) * (size_t)scratch + sizeof(RedEntry) * hgg.size() +
-                        sizeof(double) * (size_t)n_ao * n_ao * 2 +
+                        sizeof(double) * (size_t)P.p_elems * 2 +
                         sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
     if (mem_limit > 0 && need > mem_limit) return kLimit;
 
-    P.n_xc_blocks = (P.n_pts + kThreads - 1) / kThreads;
+
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
@@ -513,10 +586,11 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
-         P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) &&
-         P.npart.alloc(sizeof(double) * (P.n_xc_blocks + 1)) && P.en.alloc(sizeof(double) * 2) &&
-         P.P64.alloc(sizeof(double) * (size_t)n_ao * n_ao) &&
-         P.V.alloc(sizeof(double) * (size_t)n_ao * n_ao) && P.rho.alloc(sizeof(double4) * (size_t)n_pts) &&
+         P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_wg + 1)) &&
+         P.npart.alloc(sizeof(double) * (P.n_xc_wg + 1)) && P.en.alloc(sizeof(double) * 2 * n_mol) && upload(P.mols, P.mol_host) && upload(P.ao_mol, hao_mol) &&
+         upload(P.xc_wg, hxc) &&
+         P.P64.alloc(sizeof(double) * (size_t)P.p_elems) &&
+         P.V.alloc(sizeof(double) * (size_t)P.p_elems) && P.rho.alloc(sizeof(double4) * (size_t)n_pts) &&
          P.rho_s.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.psub64.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)) &&
          P.psub32.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail));
@@ -560,7 +634,7 @@ void resolve_profile(Plan& P) {
 // null, to P.rho per caller point (points dropped at plan time keep what
 // P.rho held).  With gather_mark the profiling mark after the gather is set.
 int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark) {
-    GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), dP, P.n_ao, P.psub64.as<double>(),
+    GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
     if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
     if (gather_mark) mark(P, s);
@@ -592,9 +666,9 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);
-    ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_rows.as<int2>(), P.n_ao,
-                  P.scratch.as<double>(), dV};
-    return launch_reduce_v(rd, P.reduce_waves, s) == hipSuccess ? kOk : kLaunchFail;
+    ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_rows.as<int2>(), P.ao_mol.as<int>(),
+                  P.mols.as<MolInfo>(), P.n_ao, P.scratch.as<double>(), dV};
+    return launch_reduce_v(rd, P.reduce_waves, P.max_mol_ao, s) == hipSuccess ? kOk : kLaunchFail;
 }
 
 // Fused RKS build on device pointers, asynchronous on stream s.
@@ -607,14 +681,16 @@ int run_build(Plan& P, int func, int prec, const double* dP, double* dV, double*
     const int rc = run_rho(P, f32 ? kRhoMixed : kRhoFp64, P.rho_s.as<double4>(), dP, s, true);
     if (rc != kOk) return rc;
     mark(P, s);
-    XcArgs xa{P.rho_s.as<double4>(), P.pts.as<double4>(), P.n_pts, func, P.wv.as<double4>(), P.epart.as<double>(),
+    XcArgs xa{P.rho_s.as<double4>(), P.pts.as<double4>(), P.xc_wg.as<int2>(), P.n_xc_wg, func, P.wv.as<double4>(),
+              P.epart.as<double>(),
               P.npart.as<double>()};
     if (launch_xc(xa, s) != hipSuccess) return kLaunchFail;
     mark(P, s);
     const int rv = run_v(P, f32, P.wv.as<double4>(), 0, dV, s, true);
     if (rv != kOk) return rv;
     mark(P, s);
-    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
+    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.mols.as<MolInfo>(), P.n_mol, den, s) !=
+        hipSuccess)
         return kLaunchFail;
     mark(P, s);
     return kOk;
@@ -644,7 +720,7 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
     if (rc == kOk) rc = run_rho(P, mode, P.rho_sb.as<double4>(), dPb, s, false);
     if (rc != kOk) return rc;
     mark(P, s);
-    XcUksArgs xa{P.rho_s.as<double4>(), P.rho_sb.as<double4>(), P.pts.as<double4>(), P.n_pts, func,
+    XcUksArgs xa{P.rho_s.as<double4>(), P.rho_sb.as<double4>(), P.pts.as<double4>(), P.xc_wg.as<int2>(), P.n_xc_wg, func,
                  P.wv.as<double4>(), P.wv_b.as<double4>(), P.epart.as<double>(), P.npart.as<double>()};
     if (launch_xc_uks(xa, s) != hipSuccess) return kLaunchFail;
     mark(P, s);
@@ -653,7 +729,8 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
     if (rc != kOk) return rc;
     mark(P, s);
     mark(P, s);
-    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.n_xc_blocks, den, s) != hipSuccess)
+    if (launch_reduce_en(P.epart.as<double>(), P.npart.as<double>(), P.mols.as<MolInfo>(), P.n_mol, den, s) !=
+        hipSuccess)
         return kLaunchFail;
     mark(P, s);
     return kOk;
@@ -703,7 +780,8 @@ int calchem_hip_vxc_upload_grid(const CalchemAoShell* shells, int n_shells, cons
     CalchemVxcOptions o;
     std::memset(&o, 0, sizeof o);
     o.memory_limit_bytes = memory_limit_bytes;
-    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyz.data(), 4, n_pts, n_ao, false, &o,
+    const std::vector<MolSpec> one{MolSpec{0, n_shells, 0, n_pts, n_ao}};
+    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyz.data(), 4, n_pts, one, false, &o,
                        grid_id);
 }
 
@@ -713,6 +791,7 @@ int calchem_hip_vxc_eval_rho(uint64_t grid_id, const double* P_mat, double* rho_
     auto plan = find_plan(grid_id);
     if (!plan || plan->n_in != n_pts || plan->n_ao != n_ao) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // batched plans: calchem_hip_vxc_build_batched*
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     const size_t nn = (size_t)n_ao * n_ao;
@@ -732,6 +811,7 @@ int calchem_hip_vxc_accum(uint64_t grid_id, const double* wv_in, double* vxc_out
     auto plan = find_plan(grid_id);
     if (!plan || plan->n_in != n_pts || plan->n_ao != n_ao) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // batched plans: calchem_hip_vxc_build_batched*
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     hipStream_t s = P.stream;
@@ -764,7 +844,22 @@ int calchem_hip_vxc_release(uint64_t grid_id) {
 int calchem_hip_vxc_create(const CalchemAoShell* shells, int n_shells, const CalchemAoPrim* prims, int n_prims,
                            const CalchemAoComp* comps, int n_comps, const double* xyzw, int n_pts, int n_ao,
                            const CalchemVxcOptions* opts, uint64_t handle) {
-    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyzw, 4, n_pts, n_ao, true, opts, handle);
+    if (n_shells <= 0 || n_pts <= 0 || n_ao <= 0) return kBadArgs;
+    const std::vector<MolSpec> one{MolSpec{0, n_shells, 0, n_pts, n_ao}};
+    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyzw, 4, n_pts, one, true, opts, handle);
+}
+
+int calchem_hip_vxc_create_batched(int n_mol, const int* mol_shell_begin, const int* mol_point_begin,
+                                   const int* mol_n_ao, const CalchemAoShell* shells, const CalchemAoPrim* prims,
+                                   int n_prims, const CalchemAoComp* comps, int n_comps, const double* xyzw,
+                                   const CalchemVxcOptions* opts, uint64_t handle) {
+    if (n_mol <= 0 || !mol_shell_begin || !mol_point_begin || !mol_n_ao) return kBadArgs;
+    std::vector<MolSpec> ms(n_mol);
+    for (int k = 0; k < n_mol; ++k)
+        ms[k] = MolSpec{mol_shell_begin[k], mol_shell_begin[k + 1], mol_point_begin[k], mol_point_begin[k + 1],
+                        mol_n_ao[k]};
+    return create_plan(shells, mol_shell_begin[n_mol], prims, n_prims, comps, n_comps, xyzw, 4,
+                       mol_point_begin[n_mol], ms, true, opts, handle);
 }
 
 int calchem_hip_vxc_build(uint64_t handle, int xc, int precision, const double* P_mat, double* V_out, double* exc,
@@ -774,6 +869,7 @@ int calchem_hip_vxc_build(uint64_t handle, int xc, int precision, const double* 
     auto plan = find_plan(handle);
     if (!plan) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // batched plans: calchem_hip_vxc_build_batched*
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     hipStream_t s = P.stream;
@@ -798,6 +894,7 @@ int calchem_hip_vxc_build_device(uint64_t handle, int xc, int precision, const d
     auto plan = find_plan(handle);
     if (!plan) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // batched plans: calchem_hip_vxc_build_batched*
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     return run_build(P, xc, precision, d_P, d_V, d_exc_nelec, static_cast<hipStream_t>(stream));
@@ -810,6 +907,7 @@ int calchem_hip_vxc_build_uks(uint64_t handle, int xc, int precision, const doub
     auto plan = find_plan(handle);
     if (!plan) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // batched plans: calchem_hip_vxc_build_batched*
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     hipStream_t s = P.stream;
@@ -846,6 +944,109 @@ int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, con
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     return run_build_uks(P, xc, precision, d_Pa, d_Pb, d_Va, d_Vb, d_exc_nelec, static_cast<hipStream_t>(stream));
+}
+
+int calchem_hip_vxc_build_batched(uint64_t handle, int xc, int precision, const double* P_packed,
+                                  double* V_packed, double* exc, double* nelec) {
+    if (!P_packed || !V_packed || !exc || handle == 0) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    hipStream_t s = P.stream;
+    const size_t ne = (size_t)P.p_elems;
+    if (hipMemcpyAsync(P.P64.p, P_packed, sizeof(double) * ne, hipMemcpyHostToDevice, s) != hipSuccess)
+        return kLaunchFail;
+    const int rc = run_build(P, xc, precision, P.P64.as<double>(), P.V.as<double>(), P.en.as<double>(), s);
+    if (rc != kOk) return rc;
+    std::vector<double> en(2 * (size_t)P.n_mol);
+    if (hipMemcpyAsync(V_packed, P.V.p, sizeof(double) * ne, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(en.data(), P.en.p, sizeof(double) * en.size(), hipMemcpyDeviceToHost, s) != hipSuccess)
+        return kLaunchFail;
+    if (hipStreamSynchronize(s) != hipSuccess) return kLaunchFail;
+    for (int k = 0; k < P.n_mol; ++k) {
+        exc[k] = en[2 * k];
+        if (nelec) nelec[k] = en[2 * k + 1];
+    }
+    return kOk;
+}
+
+int calchem_hip_vxc_build_batched_device(uint64_t handle, int xc, int precision, const double* d_P_packed,
+                                         double* d_V_packed, double* d_exc_nelec, void* stream) {
+    if (!d_P_packed || !d_V_packed || !d_exc_nelec || handle == 0) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    return run_build(P, xc, precision, d_P_packed, d_V_packed, d_exc_nelec, static_cast<hipStream_t>(stream));
+}
+
+int calchem_hip_vxc_graph_create(uint64_t handle, int xc, int precision, const double* d_P, double* d_V,
+                                 double* d_exc_nelec, uint64_t* graph_out) {
+    if (!d_P || !d_V || !d_exc_nelec || !graph_out || handle == 0) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    hipStream_t s = P.stream;
+    // One eager build first: launch attributes are set and every buffer the
+    // build touches exists before capture.
+    int rc = run_build(P, xc, precision, d_P, d_V, d_exc_nelec, s);
+    if (rc != kOk) return rc;
+    if (hipStreamSynchronize(s) != hipSuccess) return kLaunchFail;
+    const bool prof = P.prof.on;
+    P.prof.on = false;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        P.prof.on = prof;
+        return kLaunchFail;
+    }
+    rc = run_build(P, xc, precision, d_P, d_V, d_exc_nelec, s);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    P.prof.on = prof;
+    if (rc != kOk || ec != hipSuccess || !g) {
+        if (g) hipGraphDestroy(g);
+        return rc != kOk ? rc : kLaunchFail;
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess || !ex) return kLaunchFail;
+    P.graphs.push_back(ex);
+    *graph_out = P.graphs.size();
+    return kOk;
+}
+
+int calchem_hip_vxc_graph_launch(uint64_t handle, uint64_t graph, void* stream) {
+    if (handle == 0 || graph == 0) return kBadArgs;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (graph > P.graphs.size() || !P.graphs[graph - 1]) return kMismatch;
+    hipSetDevice(P.device);
+    return hipGraphLaunch(P.graphs[graph - 1], static_cast<hipStream_t>(stream)) == hipSuccess ? kOk : kLaunchFail;
+}
+
+int calchem_hip_vxc_graph_destroy(uint64_t handle, uint64_t graph) {
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    std::lock_guard<std::mutex> lk(P.mu);
+    if (graph == 0 || graph > P.graphs.size()) return kMismatch;
+    hipSetDevice(P.device);
+    if (P.graphs[graph - 1]) {
+        hipDeviceSynchronize();
+        hipGraphExecDestroy(P.graphs[graph - 1]);
+        P.graphs[graph - 1] = nullptr;
+    }
+    return kOk;
 }
 
 int calchem_hip_vxc_set_profiling(uint64_t handle, int on) {

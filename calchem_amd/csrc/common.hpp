@@ -45,8 +45,20 @@ struct RhoBlock {
     int sh_begin;  // offset into the block shell list
     int n_sh;      // shells owning the rows
     int n_prim;    // primitives of those shells
-    int pad1;
+    int mol;       // molecule of a batched plan (0 otherwise)
     long long psub;  // offset of the block's gathered density (m_pad x m_pad, row-major)
+};
+
+// One molecule of a plan (a single-molecule plan has one).  Its AOs are the
+// global rows [ao_base, ao_base + n_ao); its density and V blocks sit at
+// p_off / v_off (elements) in the packed n_ao x n_ao row-major matrices; its
+// sorted points are [pt_begin, pt_begin + n_pts) and its functional-stage
+// workgroups [xc_wg_begin, xc_wg_begin + n_xc_wg).
+struct MolInfo {
+    int ao_base, n_ao;
+    long long p_off;
+    int pt_begin, n_pts;
+    int xc_wg_begin, n_xc_wg;
 };
 
 // A row as the kernels keep it in LDS: component scale, Cartesian exponents

@@ -20,10 +20,13 @@ struct RhoArgs {
     int rho_perm;
 };
 
+// Functional-stage workgroups never straddle molecules: workgroup w covers
+// sorted points [wg[w].x, wg[w].x + wg[w].y).
 struct XcArgs {
     const double4* rho;     // per sorted point
     const double4* pts;     // sorted (x, y, z, w)
-    int n_pts;
+    const int2* wg;
+    int n_wg;
     int func;
     double4* wv;            // per sorted point
     double* e_part;         // per workgroup
@@ -35,7 +38,8 @@ struct XcUksArgs {
     const double4* rho_a;   // per sorted point (rho_a, grad rho_a)
     const double4* rho_b;
     const double4* pts;
-    int n_pts;
+    const int2* wg;
+    int n_wg;
     int func;
     double4* wv_a;          // (w v_a / 2, w u_a), u_a = 2 v_gaa grad rho_a + v_gab grad rho_b
     double4* wv_b;
@@ -70,8 +74,8 @@ struct VArgs {
 struct GatherArgs {
     const RhoBlock* blocks;
     const int2* block_rows;
-    const double* P;
-    int n_ao;
+    const MolInfo* mols;
+    const double* P;        // packed per-molecule density blocks
     double* out64;          // may be null
     float* out32;           // may be null
     int out64_leading_only; // write out64 only for the block's first m64 columns
@@ -90,9 +94,11 @@ struct ReduceArgs {
     const int* ao_gbegin;   // [n_ao + 1] CSR of the entries of each AO, groups ascending
     const RedEntry* entries;
     const int2* group_rows; // per group row: (global AO, group-local shell)
-    int n_ao;
+    const int* ao_mol;      // molecule of each global AO
+    const MolInfo* mols;
+    int n_ao;               // global AO rows
     const double* scratch;
-    double* V;              // n_ao x n_ao, full symmetric
+    double* V;              // packed per-molecule V blocks, full symmetric
 };
 
 struct SigArgs {
@@ -120,9 +126,11 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);
 size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim);
 hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
                             hipStream_t s);
-int reduce_v_waves(int n_ao, size_t lds_limit);
-hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s);
-hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s);
+int reduce_v_waves(int max_mol_ao, size_t lds_limit);
+hipError_t launch_reduce_v(const ReduceArgs& a, int waves, int max_mol_ao, hipStream_t s);
+// Per molecule: (E_xc, N) = sums of its functional-stage partials, into out[2k], out[2k+1].
+hipError_t launch_reduce_en(const double* e, const double* ne, const MolInfo* mols, int n_mol, double* out,
+                            hipStream_t s);
 hipError_t launch_to_float(const double* x, float* y, long long n, hipStream_t s);
 hipError_t launch_fill_zero(double* x, long long n, hipStream_t s);
 hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, hipStream_t s);

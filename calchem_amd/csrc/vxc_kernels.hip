@@ -305,9 +305,10 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
 // the workgroup's sums of w f_xc and w rho (vxc.rs:892-920).
 __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
     __shared__ double se[kThreads], sn[kThreads];
-    const int g = blockIdx.x * kThreads + threadIdx.x;
+    const int2 wg = a.wg[blockIdx.x];
+    const int g = wg.x + threadIdx.x;
     double e = 0.0, ne = 0.0;
-    if (g < a.n_pts) {
+    if (static_cast<int>(threadIdx.x) < wg.y) {
         const double4 r = a.rho[g];
         const double w = a.pts[g].w;
         ne = w * r.x;
@@ -346,9 +347,10 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
 // unclamped densities.
 __global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
     __shared__ double se[kThreads], sn[kThreads];
-    const int g = blockIdx.x * kThreads + threadIdx.x;
+    const int2 wg = a.wg[blockIdx.x];
+    const int g = wg.x + threadIdx.x;
     double e = 0.0, ne = 0.0;
-    if (g < a.n_pts) {
+    if (static_cast<int>(threadIdx.x) < wg.y) {
         const double4 ra = a.rho_a[g], rb = a.rho_b[g];
         const double w = a.pts[g].w;
         ne = w * (ra.x + rb.x);
@@ -653,10 +655,12 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
 // mirrored (V is exactly symmetric).  Group rows are in ascending AO order, so
 // nu <= mu is lj <= li.  Deterministic for a given plan.
 __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
-    extern __shared__ double srow[];  // [W][mu + 1]
+    extern __shared__ double srow[];  // [W][local mu + 1]
     const int mu = blockIdx.x, tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int nw = blockDim.x >> 6, len = mu + 1;
+    const MolInfo mol = a.mols[a.ao_mol[mu]];
+    const int base = mol.ao_base, lmu = mu - base;
+    const int nw = blockDim.x >> 6, len = lmu + 1;
     for (int j = tid; j < nw * len; j += blockDim.x) srow[j] = 0.0;
     __syncthreads();
     double* __restrict__ my = srow + w * len;
@@ -664,24 +668,31 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
     for (int e = a.ao_gbegin[mu] + w; e < e1; e += nw) {
         const RedEntry r = a.entries[e];
         const int li = r.li_ts & 0xffff, ts = r.li_ts >> 16, mask = (1 << ts) - 1;
-        const double* __restrict__ base = a.scratch + r.base;
+        const double* __restrict__ tile = a.scratch + r.base;
         const int2* __restrict__ rows = a.group_rows + r.rows;
         for (int lj = lane; lj <= li; lj += 64)
-            my[rows[lj].x] += base[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
+            my[rows[lj].x - base] += tile[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
     }
     __syncthreads();
-    const long long n = a.n_ao;
+    const long long n = mol.n_ao;
+    double* __restrict__ V = a.V + mol.p_off;
     for (int j = tid; j < len; j += blockDim.x) {
         double v = 0.0;
         for (int c = 0; c < nw; ++c) v += srow[c * len + j];
-        a.V[mu * n + j] = v;
-        a.V[j * n + mu] = v;
+        V[lmu * n + j] = v;
+        V[j * n + lmu] = v;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_reduce_en(const double* __restrict__ e, const double* __restrict__ ne,
-                                                    int n, double* __restrict__ out) {
+// One workgroup per molecule: its functional-stage partials, summed in a
+// fixed order.
+__global__ __launch_bounds__(1024) void k_reduce_en(const double* __restrict__ e_all, const double* __restrict__ ne_all,
+                                                    const MolInfo* __restrict__ mols, double* __restrict__ out) {
     __shared__ double se[1024], sn[1024];
+    const MolInfo mol = mols[blockIdx.x];
+    const double* __restrict__ e = e_all + mol.xc_wg_begin;
+    const double* __restrict__ ne = ne_all + mol.xc_wg_begin;
+    const int n = mol.n_xc_wg;
     double x[4] = {0.0, 0.0, 0.0, 0.0}, y[4] = {0.0, 0.0, 0.0, 0.0};
     int i = threadIdx.x;
     for (; i + 3 * 1024 < n; i += 4 * 1024) {
@@ -706,8 +717,8 @@ __global__ __launch_bounds__(1024) void k_reduce_en(const double* __restrict__ e
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = se[0];
-        out[1] = sn[0];
+        out[2 * blockIdx.x] = se[0];
+        out[2 * blockIdx.x + 1] = sn[0];
     }
 }
 
@@ -718,11 +729,15 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
 
 __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
     const RhoBlock B = a.blocks[blockIdx.x];
+    const MolInfo mol = a.mols[B.mol];
+    const double* __restrict__ P = a.P + mol.p_off;
     const int mp = B.m_pad;
     for (int idx = threadIdx.x; idx < mp * mp; idx += kThreads) {
         const int i = idx / mp, j = idx % mp;
         const int ri = a.block_rows[B.ao_begin + i].x, rj = a.block_rows[B.ao_begin + j].x;
-        const double v = (ri >= 0 && rj >= 0) ? a.P[static_cast<long long>(ri) * a.n_ao + rj] : 0.0;
+        const double v = (ri >= 0 && rj >= 0)
+                             ? P[static_cast<long long>(ri - mol.ao_base) * mol.n_ao + (rj - mol.ao_base)]
+                             : 0.0;
         if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[B.psub + idx] = v;
         if (a.out32) a.out32[B.psub + idx] = static_cast<float>(v);
     }
@@ -793,14 +808,14 @@ hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hi
 }
 
 hipError_t launch_xc(const XcArgs& a, hipStream_t s) {
-    if (a.n_pts <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xc, dim3((a.n_pts + kThreads - 1) / kThreads), dim3(kThreads), 0, s, a);
+    if (a.n_wg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xc, dim3(a.n_wg), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
 hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s) {
-    if (a.n_pts <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xc_uks, dim3((a.n_pts + kThreads - 1) / kThreads), dim3(kThreads), 0, s, a);
+    if (a.n_wg <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_xc_uks, dim3(a.n_wg), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -856,12 +871,12 @@ hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_grou
 }
 
 // Waves (row copies) per reduction workgroup that fit in LDS: up to 16.
-int reduce_v_waves(int n_ao, size_t lds_limit) {
-    const size_t row = sizeof(double) * (size_t)std::max(n_ao, 1);
+int reduce_v_waves(int max_mol_ao, size_t lds_limit) {
+    const size_t row = sizeof(double) * (size_t)std::max(max_mol_ao, 1);
     return (int)std::min<size_t>(16, lds_limit / row);
 }
 
-hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s) {
+hipError_t launch_reduce_v(const ReduceArgs& a, int waves, int max_mol_ao, hipStream_t s) {
     if (a.n_ao <= 0) return hipSuccess;
     static int attr_set = 0;
     if (!attr_set) {
@@ -874,13 +889,16 @@ hipError_t launch_reduce_v(const ReduceArgs& a, int waves, hipStream_t s) {
         if (e != hipSuccess) return e;
         attr_set = 1;
     }
-    // Row mu needs (mu + 1) doubles per wave; size for the last row.
-    hipLaunchKernelGGL(k_reduce_v, dim3(a.n_ao), dim3(64 * waves), sizeof(double) * (size_t)a.n_ao * waves, s, a);
+    // Row mu needs (local mu + 1) doubles per wave; size for the largest molecule.
+    hipLaunchKernelGGL(k_reduce_v, dim3(a.n_ao), dim3(64 * waves), sizeof(double) * (size_t)max_mol_ao * waves, s,
+                       a);
     return hipGetLastError();
 }
 
-hipError_t launch_reduce_en(const double* e, const double* ne, int n, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(k_reduce_en, dim3(1), dim3(1024), 0, s, e, ne, n, out);
+hipError_t launch_reduce_en(const double* e, const double* ne, const MolInfo* mols, int n_mol, double* out,
+                            hipStream_t s) {
+    if (n_mol <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_en, dim3(n_mol), dim3(1024), 0, s, e, ne, mols, out);
     return hipGetLastError();
 }
 

@@ -59,6 +59,12 @@ class DeviceArray:
         _ok(hip().hipMemcpy(d.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
         return d
 
+    def copy_from_numpy(self, a: np.ndarray) -> None:
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError(f"{a.nbytes} bytes do not fit a {self.nbytes}-byte device array")
+        _ok(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, H2D), "hipMemcpy H2D")
+
     def to_numpy(self, shape, dtype=np.float64) -> np.ndarray:
         out = np.empty(shape, dtype=dtype)
         _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, D2H), "hipMemcpy D2H")

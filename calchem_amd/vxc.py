@@ -102,6 +102,22 @@ class RksVxcEngine:
         _native.check(rc, "calchem_hip_vxc_build_device")
         self._kernel_invocations += 2
 
+    def graph_create(self, func: str, d_P: int, d_V: int, d_exc_nelec: int, precision: str = "fp64") -> int:
+        """Capture the build on these device buffers into a HIP graph; returns its id."""
+        g = C.c_uint64()
+        rc = self._lib.calchem_hip_vxc_graph_create(self.handle, xc_id(func), PRECISIONS[precision], d_P, d_V,
+                                                    d_exc_nelec, C.byref(g))
+        _native.check(rc, "calchem_hip_vxc_graph_create")
+        return g.value
+
+    def graph_launch(self, graph: int, stream: int = 0) -> None:
+        _native.check(self._lib.calchem_hip_vxc_graph_launch(self.handle, graph, stream or None),
+                      "calchem_hip_vxc_graph_launch")
+        self._kernel_invocations += 2
+
+    def graph_destroy(self, graph: int) -> None:
+        _native.check(self._lib.calchem_hip_vxc_graph_destroy(self.handle, graph), "calchem_hip_vxc_graph_destroy")
+
     def set_profiling(self, on: bool) -> None:
         _native.check(self._lib.calchem_hip_vxc_set_profiling(self.handle, 1 if on else 0),
                       "calchem_hip_vxc_set_profiling")
@@ -163,6 +179,98 @@ class UksVxcEngine(RksVxcEngine):
         rc = self._lib.calchem_hip_vxc_build_uks_device(self.handle, xc_id(func), PRECISIONS[precision], d_Pa,
                                                         d_Pb, d_Va, d_Vb, d_exc_nelec, stream or None)
         _native.check(rc, "calchem_hip_vxc_build_uks_device")
+        self._kernel_invocations += 2
+
+
+class BatchedVxcEngine(RksVxcEngine):
+    """Many independent molecules in one plan: each build stage is one launch
+    over all their blocks.  No single reference counterpart (the reference
+    builds one molecule per RksVxcEngine, vxc.rs:429-985).
+
+    engine = BatchedVxcEngine([(basis_0, grid_0), (basis_1, grid_1), ...])
+    engine.evaluate(func, [P_0, P_1, ...]) -> [(E_xc, V, N), ...]
+    Densities / V on the device are packed: molecule k's n_k x n_k block at
+    element offset engine.p_offsets[k].
+    """
+
+    def __init__(self, systems, block_points: int = 0, group_points: int = 0, memory_limit_bytes: int = 0,
+                 mixed_exact_threshold: float = 0.0, rho_tile_points: int = 0, group_max_rows: int = 0):
+        self._lib = _native.require_device()
+        self.handle = 0
+        self._kernel_invocations = 0
+        shells_l, prims_l, comps_l, pts_l = [], [], [], []
+        shell_begin, point_begin, n_ao = [0], [0], []
+        np_, nc_ = 0, 0
+        for basis, grid in systems:
+            sh, pr, cp = _records(basis)
+            sh = sh.copy()
+            sh["prim_start"] += np_
+            sh["comp_start"] += nc_
+            np_ += len(pr)
+            nc_ += len(cp)
+            shells_l.append(sh)
+            prims_l.append(pr)
+            comps_l.append(cp)
+            pts_l.append(grid.xyzw())
+            shell_begin.append(shell_begin[-1] + len(sh))
+            point_begin.append(point_begin[-1] + grid.n_points)
+            n_ao.append(basis.n_ao)
+        shells = np.ascontiguousarray(np.concatenate(shells_l))
+        prims = np.ascontiguousarray(np.concatenate(prims_l))
+        comps = np.ascontiguousarray(np.concatenate(comps_l))
+        xyzw = np.ascontiguousarray(np.concatenate(pts_l))
+        self.n_mol = len(n_ao)
+        self.n_aos = n_ao
+        self.n_ao = int(sum(n_ao))
+        self.n_points = int(point_begin[-1])
+        self.p_offsets = np.concatenate([[0], np.cumsum([n * n for n in n_ao])]).astype(np.int64)
+        sb = np.asarray(shell_begin, dtype=np.int32)
+        pb = np.asarray(point_begin, dtype=np.int32)
+        na = np.asarray(n_ao, dtype=np.int32)
+        opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold,
+                                  rho_tile_points, group_max_rows)
+        h = _native.new_handle()
+        rc = self._lib.calchem_hip_vxc_create_batched(self.n_mol, _native.ptr(sb), _native.ptr(pb), _native.ptr(na),
+                                                      _native.ptr(shells), _native.ptr(prims), len(prims),
+                                                      _native.ptr(comps), len(comps), _native.ptr(xyzw), C.byref(opts),
+                                                      h)
+        _native.check(rc, "calchem_hip_vxc_create_batched")
+        self.handle = h
+
+    def pack(self, mats) -> np.ndarray:
+        """Concatenate per-molecule n_k x n_k matrices into the packed layout."""
+        return np.ascontiguousarray(np.concatenate([np.ascontiguousarray(m, dtype=np.float64).ravel()
+                                                    for m in mats]))
+
+    def unpack(self, packed: np.ndarray):
+        return [packed[self.p_offsets[k]:self.p_offsets[k + 1]].reshape(n, n) for k, n in enumerate(self.n_aos)]
+
+    def evaluate(self, func: str, densities, precision: str = "fp64"):
+        if len(densities) != self.n_mol:
+            raise ValueError(f"{len(densities)} densities for {self.n_mol} molecules")
+        for P, n in zip(densities, self.n_aos):
+            if np.shape(P) != (n, n):
+                raise ValueError(f"density has shape {np.shape(P)}, expected {(n, n)}")
+        Pp = self.pack(densities)
+        Vp = np.empty_like(Pp)
+        e = np.empty(self.n_mol)
+        ne = np.empty(self.n_mol)
+        rc = self._lib.calchem_hip_vxc_build_batched(self.handle, xc_id(func), PRECISIONS[precision],
+                                                     _native.ptr(Pp), _native.ptr(Vp), _native.ptr(e),
+                                                     _native.ptr(ne))
+        _native.check(rc, "calchem_hip_vxc_build_batched")
+        self._kernel_invocations += 2
+        return [(float(e[k]), V, float(ne[k])) for k, V in enumerate(self.unpack(Vp))]
+
+    def evaluate_full(self, func: str, P, precision: str = "fp64"):
+        raise TypeError("BatchedVxcEngine: use evaluate(func, [P_0, P_1, ...])")
+
+    def build_device(self, func: str, d_P: int, d_V: int, d_exc_nelec: int, stream: int = 0,
+                     precision: str = "fp64") -> None:
+        """Enqueue a batched build on packed device buffers (d_exc_nelec holds 2 n_mol doubles)."""
+        rc = self._lib.calchem_hip_vxc_build_batched_device(self.handle, xc_id(func), PRECISIONS[precision], d_P,
+                                                            d_V, d_exc_nelec, stream or None)
+        _native.check(rc, "calchem_hip_vxc_build_batched_device")
         self._kernel_invocations += 2
 
 

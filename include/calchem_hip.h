@@ -147,6 +147,38 @@ int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, con
                                      const double *d_Pb, double *d_Va, double *d_Vb, double *d_exc_nelec,
                                      void *stream);
 
+/* ---- Many molecules in one plan ----
+ * No single reference counterpart: the reference builds one molecule per
+ * RksVxcEngine (vxc.rs:429-985); this puts n_mol independent molecules into
+ * one plan so each build stage is one launch over all their blocks.
+ * Molecule k owns shells [mol_shell_begin[k], mol_shell_begin[k+1]) (their
+ * ao_offset local to the molecule; prim_start / comp_start index the shared
+ * prims / comps arrays), points [mol_point_begin[k], mol_point_begin[k+1]) of
+ * xyzw (x, y, z, w rows) and mol_n_ao[k] AOs.  Densities and Vxc matrices are
+ * packed: molecule k's n_k x n_k row-major block follows those of molecules
+ * 0..k-1.  exc / nelec receive n_mol values; d_exc_nelec[2k], [2k+1] =
+ * (E_xc, N) of molecule k. */
+int calchem_hip_vxc_create_batched(int n_mol, const int *mol_shell_begin, const int *mol_point_begin,
+                                   const int *mol_n_ao, const CalchemAoShell *shells, const CalchemAoPrim *prims,
+                                   int n_prims, const CalchemAoComp *comps, int n_comps, const double *xyzw,
+                                   const CalchemVxcOptions *opts, uint64_t handle);
+int calchem_hip_vxc_build_batched(uint64_t handle, int xc, int precision, const double *P_packed,
+                                  double *V_packed, double *exc, double *nelec);
+int calchem_hip_vxc_build_batched_device(uint64_t handle, int xc, int precision, const double *d_P_packed,
+                                         double *d_V_packed, double *d_exc_nelec, void *stream);
+
+/* ---- Captured builds ----
+ * No reference counterpart (the reference launches kernels one by one per
+ * batch): for callers that repeat the same build every SCF iteration.
+ * graph_create runs the build once eagerly, then captures the whole build on
+ * these device buffers (single-molecule or batched plan) into a HIP graph;
+ * graph_launch replays it on `stream` (the caller rewrites d_P in place
+ * between launches); graph_destroy frees it (release frees the rest). */
+int calchem_hip_vxc_graph_create(uint64_t handle, int xc, int precision, const double *d_P, double *d_V,
+                                 double *d_exc_nelec, uint64_t *graph_out);
+int calchem_hip_vxc_graph_launch(uint64_t handle, uint64_t graph, void *stream);
+int calchem_hip_vxc_graph_destroy(uint64_t handle, uint64_t graph);
+
 /* ---- Fused build (functional on the device) ----
  * Replaces the per-geometry setup of RksVxcEngine::new_with_gpu
  * (crates/calchem-dft/src/vxc.rs:429-595): xyzw[n_pts*4] = (x, y, z, weight)
