@@ -91,6 +91,9 @@ __device__ __forceinline__ V opaque(V x) {
     asm("" : "+v"(x));
     return x;
 }
+__device__ __forceinline__ double4 opaque(double4 x) {
+    return make_double4(opaque(x.x), opaque(x.y), opaque(x.z), opaque(x.w));
+}
 
 // Position in an f64 16x16x4 A tile that makes its result element v of lane
 // (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
@@ -524,7 +527,7 @@ __host__ __device__ constexpr int vgroup_pairs_per_wave() {
 // 16-byte alignment after it.
 template <typename T, int MAXT>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
-    return sizeof(double4) * 2 * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+    return sizeof(double4) * 2 * kVStage + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
            sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * (kVPts + 1);
 }
 
@@ -539,10 +542,8 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     const int T16 = (m + 15) >> 4, mp = T16 << 4;
     const int npairs = T16 * (T16 + 1) / 2;
     unsigned char* q = smem;
-    double4* sPt = reinterpret_cast<double4*>(q);
-    q += sizeof(double4) * kVPts;
-    double4* sWv = reinterpret_cast<double4*>(q);
-    q += sizeof(double4) * kVPts;
+    double4* sPtBuf = reinterpret_cast<double4*>(q);  // [kVStage points][kVStage wv]
+    q += sizeof(double4) * 2 * kVStage;
     int2* sPair = reinterpret_cast<int2*>(q);
     q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
@@ -563,8 +564,12 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
                sPrim, tid, NT);
 
-    // Pair p = ti (ti + 1) / 2 + tj (ti >= tj) goes to wave p mod NW; the
-    // LDS table holds the LDS row offsets of tiles ti and tj (in elements).
+    // Pairs p = ti (ti + 1) / 2 + tj (ti >= tj) in this order; wave w takes
+    // the contiguous range [w c, (w + 1) c), c = ceil(npairs / NW), so
+    // consecutive pairs of a wave mostly share ti and its fragments are read
+    // from LDS once.  The LDS table holds the LDS row offsets of tiles ti and
+    // tj (in elements).
+    const int per_wave = (npairs + NW - 1) / NW;
     for (int p = tid; p < npairs; p += NT) {
         int ti = 0;
         while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
@@ -576,20 +581,27 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
 #pragma unroll
     for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
 
+    // Points and wv are staged kVStage at a time (one global round trip per
+    // kVStage / kVPts steps); slots past the group's last point get w = 0 and
+    // Z is zeroed there, so they add nothing.
     for (int q0 = 0; q0 < G.n_pts; q0 += kVPts) {
-        __syncthreads();
-        if (tid < kVPts) {
-            const int g = q0 + tid;
-            if (g < G.n_pts) {
-                const int sg = G.pt_begin + g;
-                sPt[tid] = a.pts[sg];
-                sWv[tid] = a.wv[a.wv_perm ? a.perm[sg] : sg];
-            } else {
-                sPt[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
-                sWv[tid] = make_double4(0.0, 0.0, 0.0, 0.0);
+        const int qs = q0 % kVStage;
+        if (qs == 0) {
+            __syncthreads();
+            if (tid < 2 * kVStage) {
+                const int j = tid % kVStage, g = q0 + j;
+                double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+                if (g < G.n_pts) {
+                    const int sg = G.pt_begin + g;
+                    if (tid < kVStage) v = a.pts[sg];
+                    else v = a.wv[a.wv_perm ? a.perm[sg] : sg];
+                }
+                sPtBuf[tid] = v;
             }
         }
         __syncthreads();
+        const double4* sPt = sPtBuf + qs;
+        const double4* sWv = sPtBuf + kVStage + qs;
         for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
             const double4 x = sPt[p];
@@ -620,26 +632,37 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
                 }
             }
             sP[r * LD + p] = v;
-            sZ[r * LD + p] = z;
+            sZ[r * LD + p] = q0 + p < G.n_pts ? z : T(0);
         }
         __syncthreads();
+        int cur_i = -1;
+        T zi[4], pi[4];
 #pragma unroll
         for (int w = 0; w < PPW; ++w) {
-            if (wave + NW * w >= npairs) continue;
-            const int2 pr = sPair[wave + NW * w];
-            const int ri = pr.x + col * LD, rj = pr.y + col * LD;
+            const int p = wave * per_wave + w;
+            if (w >= per_wave || p >= npairs) continue;
+            const int2 pr = sPair[p];
+            if (w == 0 || pr.x != cur_i) {
+                cur_i = pr.x;
+                const int ri = pr.x + col * LD + kq;
 #pragma unroll
-            for (int k = 0; k < kVPts; k += 4) {
-                const int kk = k + kq;
-                acc[w] = M::op(sZ[ri + kk], sP[rj + kk], acc[w]);
-                acc[w] = M::op(sP[ri + kk], sZ[rj + kk], acc[w]);
+                for (int k = 0; k < 4; ++k) {
+                    zi[k] = sZ[ri + 4 * k];
+                    pi[k] = sP[ri + 4 * k];
+                }
+            }
+            const int rj = pr.y + col * LD + kq;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc[w] = M::op(zi[k], sP[rj + 4 * k], acc[w]);
+                acc[w] = M::op(pi[k], sZ[rj + 4 * k], acc[w]);
             }
         }
     }
 #pragma unroll
     for (int w = 0; w < PPW; ++w) {
-        const int p = wave + NW * w;
-        if (p >= npairs) continue;
+        const int p = wave * per_wave + w;
+        if (w >= per_wave || p >= npairs) continue;
         double* out = a.scratch + G.scratch_off + static_cast<long long>(p) * 256;
 #pragma unroll
         for (int v = 0; v < 4; ++v) out[M::row(kq, v) * 16 + col] = static_cast<double>(acc[w][v]);
