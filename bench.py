@@ -253,7 +253,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--block-points", type=int, default=0)
     ap.add_argument("--group-points", type=int, default=0)
-    ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16, 32])
+    ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16])
     ap.add_argument("--group-max-rows", type=int, default=0, choices=[0, 192, 272])
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")

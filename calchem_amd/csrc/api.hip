@@ -68,13 +68,12 @@ struct Plan {
     int n_pts = 0;      // points kept (|w| >= 1e-24), sorted into blocks
     bool has_weights = true;
     int n_blocks = 0, n_groups = 0, n_pairs = 0;
-    int n_tiles[2] = {0, 0};  // rho tiles of 32 / 16 points
+    long n_rho_tiles = 0;     // 16-point rho tiles
     int max_mpad_rho = 0, max_m_v = 0;
-    // Launch classes per granularity (32 / 16 points) and the granularity
-    // each class uses per mode (FP64 / mixed); 0 = does not fit in LDS.
-    std::vector<RhoClass> classes[2];
-    std::vector<int> class_pts[2];
-    double flops_rho = 0, flops_v = 0, flops_rho_pad[2] = {0, 0}, flops_v_pad = 0, flops_rho_fp64_rows = 0;
+    // Rho launch classes and, per mode (FP64 / mixed), whether each fits LDS.
+    std::vector<RhoClass> classes;
+    std::vector<char> class_fits[2];
+    double flops_rho = 0, flops_v = 0, flops_rho_pad = 0, flops_v_pad = 0, flops_rho_fp64_rows = 0;
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
     int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, reduce_waves = 1;
@@ -82,7 +81,7 @@ struct Plan {
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
-    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, tiles32, tiles16, groups, group_rows,
+    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, rho_tiles, groups, group_rows,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg;
     bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
@@ -99,7 +98,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &tiles32,   &tiles16,  &groups, &group_rows, &group_sh, &group_list,
+                             &block_sh, &rho_tiles, &groups, &group_rows, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg};
@@ -243,6 +242,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (opt && opt->mixed_exact_threshold > 0.0) P.mixed_threshold = opt->mixed_exact_threshold;
     const int want_pts = opt ? opt->rho_tile_points : 0;
     if (want_pts != 0 && want_pts != 16 && want_pts != 32) return kBadArgs;
+    if (want_pts == 32) return kUnsupported;  // the rho stage walks blocks 16 points at a time
     const int want_rows = opt ? opt->group_max_rows : 0;
     if (want_rows != 0 && want_rows != 192 && want_rows != 272) return kBadArgs;
     P.group_max_tiles = want_rows == 192 ? 12 : kGroupMaxTiles;
@@ -422,38 +422,35 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         P.flops_rho += 2.0 * m * (double)m * nb;
         P.flops_rho_fp64_rows += 2.0 * m * (double)big.size() * nb;
     }
-    // Tiles of 32 and of 16 points, grouped into launch classes by m_pad so
+    // Tiles of 16 points grouped into launch classes by the block's m_pad so
     // each launch sizes its LDS for similar blocks.
-    std::vector<RhoTile> tl[2];
-    for (int g = 0; g < 2; ++g) {
-        const int pts = g == 0 ? 32 : 16;
-        const int ncls = (P.max_mpad_rho + 63) / 64;
-        std::vector<std::vector<RhoTile>> bycls(std::max(ncls, 1));
-        std::vector<RhoClass> cls(std::max(ncls, 1), RhoClass{0, 0, 0, 0, 0, 0});
+    std::vector<int2> htiles;
+    {
+        const int ncls = std::max((P.max_mpad_rho + 63) / 64, 1);
+        std::vector<std::vector<int2>> bycls(ncls);
+        std::vector<RhoClass> cls(ncls, RhoClass{0, 0, 0, 0, 0, 0});
         for (int b = 0; b < P.n_blocks; ++b) {
+            const int nb = leaves[b].second - leaves[b].first;
+            hb[b].pt_begin = leaves[b].first;
+            hb[b].n_pts = nb;
             if (hb[b].m == 0) continue;
             const int c = (hb[b].m_pad - 1) / 64;
-            const int nb = leaves[b].second - leaves[b].first;
-            for (int o = 0; o < nb; o += pts) {
-                bycls[c].push_back(RhoTile{b, leaves[b].first + o, std::min(pts, nb - o), 0});
-                P.flops_rho_pad[g] += 2.0 * hb[b].m_pad * (double)hb[b].m_pad * pts;
-            }
+            for (int o = 0; o < nb; o += kRhoPts) bycls[c].push_back(make_int2(b, o));
+            P.n_rho_tiles += (nb + kRhoPts - 1) / kRhoPts;
+            P.flops_rho_pad += 2.0 * hb[b].m_pad * (double)hb[b].m_pad * round_up(nb, kRhoPts);
             cls[c].max_mpad = std::max(cls[c].max_mpad, hb[b].m_pad);
             cls[c].max_nsh = std::max(cls[c].max_nsh, hb[b].n_sh);
             cls[c].max_m64 = std::max(cls[c].max_m64, hb[b].m64);
             cls[c].max_prim = std::max(cls[c].max_prim, hb[b].n_prim);
         }
-        for (size_t c = 0; c < bycls.size(); ++c) {
+        for (int c = 0; c < ncls; ++c) {
             if (bycls[c].empty()) continue;
-            cls[c].tile_begin = static_cast<int>(tl[g].size());
+            cls[c].tile_begin = static_cast<int>(htiles.size());
             cls[c].n_tiles = static_cast<int>(bycls[c].size());
-            tl[g].insert(tl[g].end(), bycls[c].begin(), bycls[c].end());
-            P.classes[g].push_back(cls[c]);
+            htiles.insert(htiles.end(), bycls[c].begin(), bycls[c].end());
+            P.classes.push_back(cls[c]);
         }
-        P.n_tiles[g] = static_cast<int>(tl[g].size());
     }
-    std::vector<RhoTile>& t32 = tl[0];
-    std::vector<RhoTile>& t16 = tl[1];
 
     // V groups: consecutive blocks (k-d order) merged up to nb_v points and,
     // for the register-resident kernel, up to 272 rows; the group's rows are
@@ -549,22 +546,15 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             }
     }
 
-    // LDS budget: per class and mode, 32-point tiles when they fit, else 16.
+    // LDS budget per class and mode.
     int lds_max = 0;
     if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device) != hipSuccess)
         lds_max = 65536;
     P.lds_limit = static_cast<size_t>(std::max(lds_max, 65536));
-    // class_pts[mode][class]: classes are indexed alike in both granularities
-    // (same blocks), so the 32-point class list drives the choice.
     for (int mode = 0; mode < 2; ++mode) {
-        P.class_pts[mode].assign(P.classes[0].size(), 0);
-        for (size_t c = 0; c < P.classes[0].size(); ++c) {
-            // 16-point tiles by default: their smaller LDS footprint lets two
-            // workgroups share a CU, which outweighs reading each density
-            // block twice as often.
-            if (want_pts == 32 && rho_lds_bytes(mode, 32, P.classes[0][c]) <= P.lds_limit) P.class_pts[mode][c] = 32;
-            else if (rho_lds_bytes(mode, 16, P.classes[1][c]) <= P.lds_limit) P.class_pts[mode][c] = 16;
-        }
+        P.class_fits[mode].assign(P.classes.size(), 0);
+        for (size_t c = 0; c < P.classes.size(); ++c)
+            P.class_fits[mode][c] = rho_lds_bytes(mode, P.classes[c]) <= P.lds_limit;
     }
 
     if (P.n_group_list > 0 &&
@@ -581,8 +571,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
-         upload(P.tiles32, t32) &&
-         upload(P.tiles16, t16) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
+         upload(P.rho_tiles, htiles) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
@@ -640,16 +629,12 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
     if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
-    for (size_t c = 0; c < P.classes[0].size(); ++c) {
-        const int pts = P.class_pts[mode][c];
-        if (pts == 0) return kLimit;
-        const int g = pts == 32 ? 0 : 1;
-        RhoClass cl = P.classes[g][c];
+    for (size_t c = 0; c < P.classes.size(); ++c) {
+        if (!P.class_fits[mode][c]) return kLimit;
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-                   g == 0 ? P.tiles32.as<RhoTile>() : P.tiles16.as<RhoTile>(), P.blocks.as<RhoBlock>(),
-                   P.block_rows.as<int2>(), P.block_sh.as<int2>(), dP64, dPS,
-                   sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
-        if (launch_rho(ra, mode, pts, cl, s) != hipSuccess) return kLaunchFail;
+                   P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
+                   dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
+        if (launch_rho(ra, mode, P.classes[c], s) != hipSuccess) return kLaunchFail;
     }
     return kOk;
 }
@@ -1074,29 +1059,17 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats* out) {
     out->n_points = P.n_pts;
     out->n_blocks = P.n_blocks;
     out->n_groups = P.n_groups;
-    out->n_rho_tiles = P.n_tiles[0];
+    out->n_rho_tiles = P.n_rho_tiles;
     out->n_v_pairs = P.n_pairs;
     out->flops_rho = P.flops_rho;
     out->flops_v = P.flops_v;
-    double ex[2] = {0.0, 0.0};
-    int pts_used[2] = {0, 0};
-    for (int mode = 0; mode < 2; ++mode)
-        for (size_t c = 0; c < P.classes[0].size(); ++c) {
-            const int pts = P.class_pts[mode][c];
-            if (pts == 0) continue;
-            pts_used[mode] = std::max(pts_used[mode], pts);
-            const RhoClass& cl = P.classes[pts == 32 ? 0 : 1][c];
-            (void)cl;
-        }
-    ex[0] = P.flops_rho_pad[pts_used[0] == 16 ? 1 : 0];
-    ex[1] = P.flops_rho_pad[pts_used[1] == 16 ? 1 : 0];
-    out->flops_rho_executed_fp64 = ex[0];
-    out->flops_rho_executed_fp32 = ex[1];
+    out->flops_rho_executed_fp64 = P.flops_rho_pad;
+    out->flops_rho_executed_fp32 = P.flops_rho_pad;
     out->flops_v_executed = P.flops_v_pad;
     out->max_m_rho = P.max_mpad_rho;
     out->max_m_v = P.max_m_v;
-    out->rho_points_fp64 = pts_used[0];
-    out->rho_points_fp32 = pts_used[1];
+    out->rho_points_fp64 = kRhoPts;
+    out->rho_points_fp32 = kRhoPts;
     out->flops_rho_fp64_rows = P.flops_rho_fp64_rows;
     out->device_bytes = P.device_bytes();
     out->lds_bytes_per_block = P.lds_limit;

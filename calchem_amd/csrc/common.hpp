@@ -47,6 +47,8 @@ struct RhoBlock {
     int n_prim;    // primitives of those shells
     int mol;       // molecule of a batched plan (0 otherwise)
     long long psub;  // offset of the block's gathered density (m_pad x m_pad, row-major)
+    int pt_begin;  // first sorted point
+    int n_pts;
 };
 
 // One molecule of a plan (a single-molecule plan has one).  Its AOs are the
@@ -69,14 +71,6 @@ struct RowL {
     int sh;
 };
 
-// One workgroup of the rho kernel: up to kRhoPts consecutive sorted points
-// of one block.
-struct RhoTile {
-    int block;
-    int pt_begin;  // sorted point index
-    int n_pts;
-    int pad;
-};
 
 // V-stage group (a union of neighbouring rho blocks).  Its rows are the
 // union of the blocks' AOs in ascending order; V tiles of (1 << tile_shift)
@@ -99,6 +93,7 @@ constexpr int kVTile = 64;       // tile of the tile-pair V kernel
 constexpr int kVPts = 16;        // points per V-stage step (MFMA k extent 4 x 4)
 constexpr int kVStage = 64;      // points (and wv) loaded into LDS per round trip in the V stage
 constexpr int kThreads = 256;    // 4 wavefronts of 64
+constexpr int kRhoPts = 16;      // points per rho-stage tile (MFMA N extent)
 constexpr int kGroupMaxTiles = 17;   // default 16-row tiles per register-resident V group (272 rows)
 // Elements after the last gathered density block: the rho kernel's look-ahead
 // loads may read up to 3 chunks of 16 past a row's end (values unused).

@@ -9,7 +9,7 @@ struct RhoArgs {
     const double2* prims;   // (exponent, c~ N(alpha; L00))
     const DevAo* aos;
     const double4* pts;     // sorted (x, y, z, w)
-    const RhoTile* tiles;   // this launch's tiles
+    const int2* tiles;      // this launch's tiles: (block, first point within the block)
     const RhoBlock* blocks;
     const int2* block_rows; // per block row: (global AO, block-local shell) or (-1, 0)
     const int2* block_sh;   // per block shell: (global shell, block-local primitive offset)
@@ -49,7 +49,7 @@ struct XcUksArgs {
 
 // Launch class of rho tiles: blocks of similar size share one LDS budget.
 struct RhoClass {
-    int tile_begin, n_tiles;  // range in the tile list of one point granularity
+    int tile_begin, n_tiles;  // range in the class-ordered tile list
     int max_mpad, max_nsh, max_m64, max_prim;
 };
 
@@ -114,10 +114,10 @@ struct SigArgs {
 };
 
 // Rho kernel variants: FP64 throughout, or mixed (leading rows FP64, the
-// rest FP32).  pts = 32 or 16 points per workgroup.
+// rest FP32).
 enum RhoMode { kRhoFp64 = 0, kRhoMixed = 1 };
-size_t rho_lds_bytes(int mode, int pts, const RhoClass& c);
-hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hipStream_t s);
+size_t rho_lds_bytes(int mode, const RhoClass& c);
+hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s);
 hipError_t launch_xc(const XcArgs& a, hipStream_t s);
 hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);

@@ -70,34 +70,12 @@ struct Vec4<float> {
 // ---------------------------------------------------------------------------
 // rho stage
 // ---------------------------------------------------------------------------
-// Row stride (elements) of the LDS value tables: rows of one MFMA k-step land
-// on disjoint LDS banks for both halves of a wavefront.
-template <int PTS>
-__host__ __device__ constexpr int rho_ldp() {
-    return PTS == 32 ? 48 : 16;
-}
-
 // Dynamic LDS of one rho workgroup, carved in this order (every piece is a
 // multiple of 16 bytes): points, per-wave sums, radial table [shell][point],
 // shells, primitives, rows, FP64 leading-row values, row values.
-template <int PTS>
 __host__ __device__ constexpr size_t rho_lds_fixed() {
-    return sizeof(double4) * PTS + sizeof(double) * 16 * PTS;
+    return sizeof(double4) * kRhoPts + sizeof(double) * 16 * kRhoPts;
 }
-
-// x unchanged, through a register the compiler cannot see into.
-template <typename V>
-__device__ __forceinline__ V opaque(V x) {
-    asm("" : "+v"(x));
-    return x;
-}
-__device__ __forceinline__ double4 opaque(double4 x) {
-    return make_double4(opaque(x.x), opaque(x.y), opaque(x.z), opaque(x.w));
-}
-
-// Position in an f64 16x16x4 A tile that makes its result element v of lane
-// (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
-__device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >> 2); }
 
 // Workgroups are dispatched round robin over the 8 XCDs (workgroup i to XCD
 // i mod 8), each with its own L2.  This maps workgroup i to work item
@@ -111,17 +89,40 @@ __device__ __forceinline__ int xcd_order(int i, int n) {
     return x * q + min(x, r) + k;
 }
 
-template <typename TS, int PTS, bool HYB>
-__global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
+// Position in an f64 16x16x4 A tile that makes its result element v of lane
+// (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
+__device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >> 2); }
+
+// x unchanged, through a register the compiler cannot see into.
+template <typename V>
+__device__ __forceinline__ V opaque(V x) {
+    asm("" : "+v"(x));
+    return x;
+}
+
+// One workgroup (4 waves) per tile of 16 points of a rho block.  The block's
+// rows, shells and primitives are staged in LDS; radial sums once per (shell,
+// point); then wave w evaluates (phi, grad phi) for its 16-row tiles
+// w, w + 4, ... in exactly the lanes and slots where the MFMA result T of
+// those rows will land (lane (j, q), slot v: row Mma<TS>::row(q, v), point
+// j); phi goes to LDS as the MFMA B operand, (phi, grad phi) of the first
+// row tiles (3 in FP64, 2 in the mixed build, which holds FP64 leading-row
+// accumulators too) stay in registers for the epilogue (later row tiles
+// recompute them).  T = P_sub Phi on MFMA (density rows from L2, loaded two
+// chunks ahead), rho += phi T and grad rho += 2 grad phi T, summed over rows
+// with cross-lane shuffles and then over the waves.
+
+template <typename TS, bool HYB>
+__global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int LDP = rho_ldp<PTS>();
-    const RhoTile tile = a.tiles[xcd_order(blockIdx.x, gridDim.x)];
-    const RhoBlock blk = a.blocks[tile.block];
-    const int m = blk.m, mp = blk.m_pad, np = tile.n_pts, nsh = blk.n_sh;
+    constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = HYB ? 2 : 3;
+    const int2 tile = a.tiles[xcd_order(blockIdx.x, gridDim.x)];  // (block, first point within it)
+    const RhoBlock blk = a.blocks[tile.x];
+    const int m = blk.m, mp = blk.m_pad, nsh = blk.n_sh, T16 = blk.m_pad >> 4;
     const int m64 = HYB ? blk.m64 : 0;
     unsigned char* q = smem;
     double4* sPt = reinterpret_cast<double4*>(q);
-    q += sizeof(double4) * PTS;
+    q += sizeof(double4) * kRhoPts;
     double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 sums][PTS]
     q += sizeof(double) * 16 * PTS;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
@@ -136,170 +137,170 @@ __global__ __launch_bounds__(kThreads) void k_rho(RhoArgs a) {
     q += HYB ? sizeof(double) * m64 * LDP : 0;
     TS* sPhi = reinterpret_cast<TS*>(q);  // [mp][LDP]
 
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+    using M = Mma<TS>;
+    const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
     stage_meta(a.block_rows + blk.ao_begin, m, mp, a.block_sh + blk.sh_begin, nsh, a.shells, a.prims, a.aos, sRow,
                sSh, sPrim, tid, kThreads);
-    if (tid < PTS) sPt[tid] = tid < np ? a.pts[tile.pt_begin + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
+    const int t0 = tile.y, np = min(PTS, blk.n_pts - t0);
+    if (tid < PTS) sPt[tid] = tid < np ? a.pts[blk.pt_begin + t0 + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
+    {
 
-    // Radial sums once per (shell, point).
-    for (int idx = tid; idx < nsh * PTS; idx += kThreads) {
-        const int sh = idx / PTS, p = idx % PTS;
-        double2 rr = make_double2(0.0, 0.0);
-        if (p < np) {
-            const double4 x = sPt[p];
-            rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
-        }
-        sRad[idx] = rr;
-    }
-    __syncthreads();
-
-    // AO values of every row at every point of the tile.
-    for (int idx = tid; idx < mp * PTS; idx += kThreads) {
-        const int r = idx / PTS, p = idx % PTS;
-        const RowL row = sRow[r];
-        double v = 0.0;
-        if (row.sh >= 0) {
-            const double4 x = sPt[p];
-            v = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + p]).v;
-        }
-        sPhi[r * LDP + p] = static_cast<TS>(v);
-        if (HYB && r < m64) sPhi64[r * LDP + p] = v;
-    }
-    __syncthreads();
-
-    using M = Mma<TS>;
-    constexpr int NH = PTS / 16;
-    const int col = lane & 15, kq = lane >> 4;
-    const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
-    double sr[NH], sx[NH], sy[NH], sz[NH];
-#pragma unroll
-    for (int h = 0; h < NH; ++h) sr[h] = sx[h] = sy[h] = sz[h] = 0.0;
-
-    for (int it = wave; it < (mp >> 4); it += 4) {
-        const int i0 = it << 4;
-        typename M::acc acc[NH];
-#pragma unroll
-        for (int h = 0; h < NH; ++h) acc[h] = M::zero();
-        v4d acc64[NH];
-        if constexpr (HYB) {
-#pragma unroll
-            for (int h = 0; h < NH; ++h) acc64[h] = Mma<double>::zero();
-            // Leading rows (large |phi|) in FP64 with the FP64 density block,
-            // four k-steps per round with their loads issued together.
-            const double* __restrict__ Arow = a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
-            int k = 0;
-            for (; k + 16 <= m64; k += 16) {
-                double av[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) av[s] = Arow[k + 4 * s + kq];
-#pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int h = 0; h < NH; ++h)
-                        acc64[h] = Mma<double>::op(av[s], sPhi64[(k + 4 * s + kq) * LDP + h * 16 + col], acc64[h]);
+        // Radial sums once per (shell, point).
+        for (int idx = tid; idx < nsh * PTS; idx += kThreads) {
+            const int sh = idx / PTS, p = idx % PTS;
+            double2 rr = make_double2(0.0, 0.0);
+            if (p < np) {
+                const double4 x = sPt[p];
+                rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
             }
-            for (; k < m64; k += 4) {
-                const int kk = k + kq;
-                const double av = Arow[kk];
+            sRad[idx] = rr;
+        }
+        __syncthreads();
+
+        // (phi, grad phi) of row Mma::row(kq, v) of row tile it at point col.
+        auto eval = [&](int it, int v) {
+            const int r = (it << 4) + M::row(kq, v);
+            const RowL row = sRow[r];
+            AoVal f{0.0, 0.0, 0.0, 0.0};
+            if (row.sh >= 0 && col < np) {
+                const double4 x = sPt[col];
+                f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + col]);
+            }
+            return f;
+        };
+        AoVal cache[kRhoCached][4];
 #pragma unroll
-                for (int h = 0; h < NH; ++h)
-                    acc64[h] = Mma<double>::op(av, sPhi64[kk * LDP + h * 16 + col], acc64[h]);
+        for (int j = 0; j < kRhoCached; ++j) {
+            const int it = wave + 4 * j;
+            if (it < T16) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int r = (it << 4) + M::row(kq, v);
+                    cache[j][v] = eval(it, v);
+                    sPhi[r * LDP + col] = static_cast<TS>(cache[j][v].v);
+                    if (HYB && r < m64) sPhi64[r * LDP + col] = cache[j][v].v;
+                }
             }
         }
-        {
-            // K is visited in chunks of 16 whose slots are permuted so that
-            // lane (i, q) reads four consecutive density entries per load:
-            // step s of a chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].
-            // Density loads run two chunks ahead (reads past the row end land
-            // in the next row or the buffer tail and are not used).
+        for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int r = (it << 4) + M::row(kq, v);
+                const double fv = eval(it, v).v;
+                sPhi[r * LDP + col] = static_cast<TS>(fv);
+                if (HYB && r < m64) sPhi64[r * LDP + col] = fv;
+            }
+        }
+        __syncthreads();
+
+        // T for row tile it: this lane ends up with T[row(kq, v)][col].
+        auto contract = [&](int it, typename M::acc& acc, v4d& acc64) {
+            const int i0 = it << 4;
+            acc = M::zero();
+            if constexpr (HYB) {
+                acc64 = Mma<double>::zero();
+                // Leading rows (large |phi|) in FP64 with the FP64 density
+                // block, four k-steps per round with their loads issued together.
+                const double* __restrict__ Arow =
+                    a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
+                int k = 0;
+                for (; k + 16 <= m64; k += 16) {
+                    double av[4];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) av[s] = Arow[k + 4 * s + kq];
+#pragma unroll
+                    for (int s = 0; s < 4; ++s)
+                        acc64 = Mma<double>::op(av[s], sPhi64[(k + 4 * s + kq) * LDP + col], acc64);
+                }
+                for (; k < m64; k += 4) acc64 = Mma<double>::op(Arow[k + kq], sPhi64[(k + kq) * LDP + col], acc64);
+            }
+            // K in chunks of 16 whose slots are permuted so that lane (i, q)
+            // reads four consecutive density entries per load: step s of a
+            // chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].  Density
+            // loads run two chunks ahead (reads past the row end land in the
+            // next row or the buffer tail and are not used); the first two
+            // enter the rotation through an empty asm so that the compiler
+            // cannot fold the rotated registers back into one load at the
+            // point of use.
             const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>(i0 + col) * mp;
             int k = m64;
-            for (; k < mp && (k & 15); k += 4) {
-                const int kk = k + kq;
-                const TS av = Arow[kk];
-#pragma unroll
-                for (int h = 0; h < NH; ++h) acc[h] = M::op(av, sPhi[kk * LDP + h * 16 + col], acc[h]);
-            }
-            // The first two chunks enter the rotation through an empty asm:
-            // with a non-load among the incoming values the compiler cannot
-            // fold the rotated registers back into one load at the point of
-            // use, so each load is issued two chunks before it is consumed.
+            for (; k < mp && (k & 15); k += 4) acc = M::op(Arow[k + kq], sPhi[(k + kq) * LDP + col], acc);
             using V4 = typename Vec4<TS>::type;
             V4 a0 = opaque(*reinterpret_cast<const V4*>(Arow + k + 4 * kq));
             V4 a1 = opaque(*reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq));
             for (; k < mp; k += 16) {
                 const V4 a2 = *reinterpret_cast<const V4*>(Arow + k + 32 + 4 * kq);
                 const int kb = k + 4 * kq;
-                TS b[4][NH];
+                TS b[4];
 #pragma unroll
-                for (int s = 0; s < 4; ++s)
-#pragma unroll
-                    for (int h = 0; h < NH; ++h) b[s][h] = sPhi[(kb + s) * LDP + h * 16 + col];
-#pragma unroll
-                for (int h = 0; h < NH; ++h) {
-                    acc[h] = M::op(a0[0], b[0][h], acc[h]);
-                    acc[h] = M::op(a0[1], b[1][h], acc[h]);
-                    acc[h] = M::op(a0[2], b[2][h], acc[h]);
-                    acc[h] = M::op(a0[3], b[3][h], acc[h]);
-                }
+                for (int s = 0; s < 4; ++s) b[s] = sPhi[(kb + s) * LDP + col];
+                acc = M::op(a0[0], b[0], acc);
+                acc = M::op(a0[1], b[1], acc);
+                acc = M::op(a0[2], b[2], acc);
+                acc = M::op(a0[3], b[3], acc);
                 a0 = a1;
                 a1 = a2;
             }
-        }
-        // Epilogue: this lane holds T[i0 + row(kq, v)][h*16 + col].
+        };
+        double sr = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+        auto accumulate = [&](const typename M::acc& acc, const v4d& acc64, int v, const AoVal& f) {
+            double t = static_cast<double>(acc[v]);
+            if constexpr (HYB) t += acc64[v];
+            sr += f.v * t;
+            sx += f.gx * t;
+            sy += f.gy * t;
+            sz += f.gz * t;
+        };
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const RowL row = sRow[i0 + M::row(kq, v)];
-            if (row.sh < 0) continue;
-            const DevShell sh = sSh[row.sh];
+        for (int j = 0; j < kRhoCached; ++j) {
+            const int it = wave + 4 * j;
+            if (it < T16) {
+                typename M::acc acc;
+                v4d acc64;
+                contract(it, acc, acc64);
 #pragma unroll
-            for (int h = 0; h < NH; ++h) {
-                const int p = h * 16 + col;
-                if (p >= np) continue;
-                double t = static_cast<double>(acc[h][v]);
-                if constexpr (HYB) t += acc64[h][v];
-                const double4 x = sPt[p];
-                const AoVal f = ao_from_radial_row(sh, row, x.x, x.y, x.z, sRad[row.sh * PTS + p]);
-                sr[h] += f.v * t;
-                sx[h] += f.gx * t;
-                sy[h] += f.gy * t;
-                sz[h] += f.gz * t;
+                for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, cache[j][v]);
             }
         }
-    }
-    // Sum over the four row groups of the wave, then over the waves.
+        for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
+            typename M::acc acc;
+            v4d acc64;
+            contract(it, acc, acc64);
 #pragma unroll
-    for (int h = 0; h < NH; ++h) {
-        sr[h] += __shfl_xor(sr[h], 16);
-        sr[h] += __shfl_xor(sr[h], 32);
-        sx[h] += __shfl_xor(sx[h], 16);
-        sx[h] += __shfl_xor(sx[h], 32);
-        sy[h] += __shfl_xor(sy[h], 16);
-        sy[h] += __shfl_xor(sy[h], 32);
-        sz[h] += __shfl_xor(sz[h], 16);
-        sz[h] += __shfl_xor(sz[h], 32);
+            for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, eval(it, v));
+        }
+        // Sum over the four row groups of the wave, then over the waves.
+        sr += __shfl_xor(sr, 16);
+        sr += __shfl_xor(sr, 32);
+        sx += __shfl_xor(sx, 16);
+        sx += __shfl_xor(sx, 32);
+        sy += __shfl_xor(sy, 16);
+        sy += __shfl_xor(sy, 32);
+        sz += __shfl_xor(sz, 16);
+        sz += __shfl_xor(sz, 32);
         if (kq == 0) {
-            const int p = h * 16 + col;
-            sRed[(wave * 4 + 0) * PTS + p] = sr[h];
-            sRed[(wave * 4 + 1) * PTS + p] = sx[h];
-            sRed[(wave * 4 + 2) * PTS + p] = sy[h];
-            sRed[(wave * 4 + 3) * PTS + p] = sz[h];
+            sRed[(wave * 4 + 0) * PTS + col] = sr;
+            sRed[(wave * 4 + 1) * PTS + col] = sx;
+            sRed[(wave * 4 + 2) * PTS + col] = sy;
+            sRed[(wave * 4 + 3) * PTS + col] = sz;
         }
-    }
-    __syncthreads();
-    if (tid < np) {
-        const int p = tid;
-        double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+        __syncthreads();
+        if (tid < np) {
+            double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
 #pragma unroll
-        for (int w = 0; w < 4; ++w) {
-            rho += sRed[(w * 4 + 0) * PTS + p];
-            gx += sRed[(w * 4 + 1) * PTS + p];
-            gy += sRed[(w * 4 + 2) * PTS + p];
-            gz += sRed[(w * 4 + 3) * PTS + p];
+            for (int w = 0; w < 4; ++w) {
+                rho += sRed[(w * 4 + 0) * PTS + tid];
+                gx += sRed[(w * 4 + 1) * PTS + tid];
+                gy += sRed[(w * 4 + 2) * PTS + tid];
+                gz += sRed[(w * 4 + 3) * PTS + tid];
+            }
+            const int g = blk.pt_begin + t0 + tid;
+            a.rho_out[a.rho_perm ? a.perm[g] : g] = make_double4(rho, 2.0 * gx, 2.0 * gy, 2.0 * gz);
         }
-        const int g = tile.pt_begin + p;
-        a.rho_out[a.rho_perm ? a.perm[g] : g] = make_double4(rho, 2.0 * gx, 2.0 * gy, 2.0 * gz);
     }
 }
 
@@ -794,16 +795,16 @@ __global__ void k_significance(SigArgs a) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-template <typename TS, int PTS, bool HYB>
+template <typename TS, bool HYB>
 static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
-    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, PTS, c);
+    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, c);
     static int attr_set = 0;
     if (!attr_set) {
         int dev = 0, lim = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, PTS, HYB>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, HYB>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
         attr_set = 1;
@@ -811,23 +812,20 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
     if (c.n_tiles <= 0) return hipSuccess;
     RhoArgs b = a;
     b.tiles = a.tiles + c.tile_begin;
-    hipLaunchKernelGGL((k_rho<TS, PTS, HYB>), dim3(c.n_tiles), dim3(kThreads), lds, s, b);
+    hipLaunchKernelGGL((k_rho<TS, HYB>), dim3(c.n_tiles), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
-size_t rho_lds_bytes(int mode, int pts, const RhoClass& c) {
-    const size_t fixed = pts == 32 ? rho_lds_fixed<32>() : rho_lds_fixed<16>();
-    const size_t ldp = pts == 32 ? rho_ldp<32>() : rho_ldp<16>();
+size_t rho_lds_bytes(int mode, const RhoClass& c) {
     const size_t ts = mode == kRhoMixed ? sizeof(float) : sizeof(double);
-    return fixed + sizeof(double2) * (size_t)c.max_nsh * pts + sizeof(DevShell) * (size_t)c.max_nsh +
+    return rho_lds_fixed() + sizeof(double2) * (size_t)c.max_nsh * kRhoPts + sizeof(DevShell) * (size_t)c.max_nsh +
            sizeof(double2) * (size_t)c.max_prim + sizeof(RowL) * (size_t)c.max_mpad +
-           (mode == kRhoMixed ? sizeof(double) * (size_t)c.max_m64 * ldp : 0) + ts * (size_t)c.max_mpad * ldp;
+           (mode == kRhoMixed ? sizeof(double) * (size_t)c.max_m64 * kRhoPts : 0) +
+           ts * (size_t)c.max_mpad * kRhoPts;
 }
 
-hipError_t launch_rho(const RhoArgs& a, int mode, int pts, const RhoClass& c, hipStream_t s) {
-    if (mode == kRhoFp64)
-        return pts == 32 ? launch_rho_t<double, 32, false>(a, c, s) : launch_rho_t<double, 16, false>(a, c, s);
-    return pts == 32 ? launch_rho_t<float, 32, true>(a, c, s) : launch_rho_t<float, 16, true>(a, c, s);
+hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s) {
+    return mode == kRhoFp64 ? launch_rho_t<double, false>(a, c, s) : launch_rho_t<float, true>(a, c, s);
 }
 
 hipError_t launch_xc(const XcArgs& a, hipStream_t s) {

@@ -78,7 +78,7 @@ typedef struct {
     int32_t group_points;          /* target points per V-stage group (default 1024) */
     size_t memory_limit_bytes;     /* 0 = no limit */
     double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.25) */
-    int32_t rho_tile_points;       /* points per rho workgroup: 0 = default (16), 16 or 32 */
+    int32_t rho_tile_points;       /* points per rho tile: 0 or 16 (other values: -1 / -7) */
     int32_t group_max_rows;        /* AO rows per register-resident V group: 0 = default (272), 192 or 272 */
     int32_t reserved[2];
 } CalchemVxcOptions;
