@@ -75,7 +75,7 @@ enum {
 
 typedef struct {
     int32_t block_points;          /* max points per rho-stage block (default 256) */
-    int32_t group_points;          /* target points per V-stage group (default 1024) */
+    int32_t group_points;          /* target points per V-stage group (default 384) */
     size_t memory_limit_bytes;     /* 0 = no limit */
     double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.25) */
     int32_t rho_tile_points;       /* points per rho tile: 0 or 16 (other values: -1 / -7) */
