@@ -1,0 +1,80 @@
+"""Pick block / group shapes for a geometry by timing builds on the device.
+
+The rho stage works on k-d blocks of at most `block_points` points and the V
+stage on groups of neighbouring blocks of at most `group_points` points and
+`group_max_rows` AO rows (register-resident up to that many rows).  Which
+shape is fastest depends on the basis density and the grid, so for a
+geometry that will be built many times (every SCF iteration) it pays to try
+a few plans once and keep the fastest: each candidate plan is built, warmed
+up, and timed over `steps` builds with HIP events on one stream; the fastest
+candidate whose E_xc agrees with the first candidate's to `e_tol` wins.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import Iterable, List, Optional, Tuple
+
+import numpy as np
+
+from . import hip_runtime as hr
+from .basis import BasisSet
+from .grid import MolecularGrid
+from .vxc import RksVxcEngine
+
+# (block_points, group_points, group_max_rows); 0 = library default.
+DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
+    (256, 384, 0), (256, 256, 0), (256, 512, 0), (256, 1024, 0), (192, 384, 0), (128, 384, 0),
+    (256, 1024, 192),
+)
+
+
+@dataclass
+class TuneResult:
+    block_points: int
+    group_points: int
+    group_max_rows: int
+    ms_per_build: float
+    table: List[Tuple[Tuple[int, int, int], float, float]]  # (candidate, ms, E_xc)
+
+    def engine(self, basis: BasisSet, grid: MolecularGrid, **kw) -> RksVxcEngine:
+        return RksVxcEngine(basis, grid, block_points=self.block_points, group_points=self.group_points,
+                            group_max_rows=self.group_max_rows, **kw)
+
+
+def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, precision: str = "fp64",
+             candidates: Optional[Iterable[Tuple[int, int, int]]] = None, steps: int = 5, warmup: int = 1,
+             e_tol: Optional[float] = None) -> TuneResult:
+    """e_tol: largest accepted |E_xc - E_xc(first candidate)|; default 1e-9 Eh for
+    FP64 builds, 1e-7 Eh for mixed ones (their rounding depends on the blocking)."""
+    if e_tol is None:
+        e_tol = 1e-9 if precision == "fp64" else 1e-7
+    cands = list(candidates or DEFAULT_CANDIDATES)
+    if not cands:
+        raise ValueError("no candidates")
+    n = basis.n_ao
+    stream = hr.Stream()
+    dP = hr.DeviceArray.from_numpy(np.ascontiguousarray(P, dtype=np.float64))
+    dV = hr.DeviceArray(n * n * 8)
+    den = hr.DeviceArray(16)
+    table = []
+    e_first = None
+    for bp, gp, gr in cands:
+        eng = RksVxcEngine(basis, grid, block_points=bp, group_points=gp, group_max_rows=gr)
+        for _ in range(max(warmup, 1)):
+            eng.build_device(func, dP.address, dV.address, den.address, stream.address, precision)
+        stream.synchronize()
+        e = float(den.to_numpy((2,))[0])
+        ev0, ev1 = hr.Event(), hr.Event()
+        ev0.record(stream)
+        for _ in range(steps):
+            eng.build_device(func, dP.address, dV.address, den.address, stream.address, precision)
+        ev1.record(stream)
+        ev1.synchronize()
+        ms = ev0.elapsed_ms(ev1) / steps
+        eng.close()
+        if e_first is None:
+            e_first = e
+        table.append(((bp, gp, gr), ms, e))
+    ok = [t for t in table if abs(t[2] - e_first) <= e_tol]
+    (bp, gp, gr), ms, _ = min(ok, key=lambda t: t[1])
+    return TuneResult(bp, gp, gr, ms, table)

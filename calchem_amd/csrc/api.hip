@@ -81,7 +81,7 @@ struct Plan {
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
-    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, rho_tiles, groups, group_rows,
+    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, rho_tiles, groups, group_rows, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg;
     bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
@@ -98,7 +98,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &rho_tiles, &groups, &group_rows, &group_sh, &group_list,
+                             &block_sh, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg};
@@ -523,6 +523,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         b = e;
     }
     P.n_groups = static_cast<int>(hg.size());
+    std::vector<int> hgao(hgrows.size());
+    for (size_t i = 0; i < hgrows.size(); ++i) hgao[i] = hgrows[i].x;
     P.n_pairs = static_cast<int>(hpairs.size());
     P.n_group_list = static_cast<int>(hglist.size());
     // CSR: for each AO the groups holding it, ascending, with its row there.
@@ -571,7 +573,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
-         upload(P.rho_tiles, htiles) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
+         upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
@@ -651,7 +653,7 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);
-    ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_rows.as<int2>(), P.ao_mol.as<int>(),
+    ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_ao.as<int>(), P.ao_mol.as<int>(),
                   P.mols.as<MolInfo>(), P.n_ao, P.scratch.as<double>(), dV};
     return launch_reduce_v(rd, P.reduce_waves, P.max_mol_ao, s) == hipSuccess ? kOk : kLaunchFail;
 }

@@ -93,7 +93,7 @@ struct RedEntry {
 struct ReduceArgs {
     const int* ao_gbegin;   // [n_ao + 1] CSR of the entries of each AO, groups ascending
     const RedEntry* entries;
-    const int2* group_rows; // per group row: (global AO, group-local shell)
+    const int* group_ao;    // per group row: its global AO
     const int* ao_mol;      // molecule of each global AO
     const MolInfo* mols;
     int n_ao;               // global AO rows

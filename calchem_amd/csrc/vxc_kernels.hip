@@ -693,9 +693,9 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
         const RedEntry r = a.entries[e];
         const int li = r.li_ts & 0xffff, ts = r.li_ts >> 16, mask = (1 << ts) - 1;
         const double* __restrict__ tile = a.scratch + r.base;
-        const int2* __restrict__ rows = a.group_rows + r.rows;
+        const int* __restrict__ rows = a.group_ao + r.rows;
         for (int lj = lane; lj <= li; lj += 64)
-            my[rows[lj].x - base] += tile[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
+            my[rows[lj] - base] += tile[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
     }
     __syncthreads();
     const long long n = mol.n_ao;
