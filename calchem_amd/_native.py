@@ -56,6 +56,7 @@ EXPORTS = [
     "calchem_hip_vxc_build_device", "calchem_hip_vxc_build_uks", "calchem_hip_vxc_build_uks_device",
     "calchem_hip_vxc_create_batched", "calchem_hip_vxc_build_batched", "calchem_hip_vxc_build_batched_device",
     "calchem_hip_vxc_graph_create", "calchem_hip_vxc_graph_launch", "calchem_hip_vxc_graph_destroy",
+    "calchem_hip_vxc_set_atoms", "calchem_hip_vxc_xc_gradient",
     "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
     "calchem_hip_grid_build",
 ]
@@ -94,6 +95,8 @@ def load() -> C.CDLL:
     L.calchem_hip_vxc_graph_create.argtypes = [u64, i32, i32, dp, dp, dp, C.POINTER(C.c_uint64)]
     L.calchem_hip_vxc_graph_launch.argtypes = [u64, u64, vp]
     L.calchem_hip_vxc_graph_destroy.argtypes = [u64, u64]
+    L.calchem_hip_vxc_set_atoms.argtypes = [u64, i32, vp, vp, vp, vp]
+    L.calchem_hip_vxc_xc_gradient.argtypes = [u64, i32, vp, vp, vp]
     L.calchem_hip_vxc_set_profiling.argtypes = [u64, i32]
     L.calchem_hip_vxc_stats.argtypes = [u64, C.POINTER(VxcStats)]
     L.calchem_hip_grid_build.restype = C.c_long

@@ -14,7 +14,7 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcalchem_hip.so")
-SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip"]
+SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip"]
 ARCH = os.environ.get("CALCHEM_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

@@ -175,6 +175,67 @@ __device__ __forceinline__ AoValF ao_from_radial_row_f(const DevShell& s, const 
     return o;
 }
 
+// For the nuclear gradient (eval_aos_hess, ao_grid.rs:234-299): radial sums
+// R0, R1, R2 = 4 sum_p a_p^2 c~_p N_p e^{-a_p r^2}, zero when the shell is
+// cut off or all three fall below 1e-18; no per-AO drop rule.
+struct Rad3 {
+    double r0, r1, r2;
+};
+
+__device__ __forceinline__ Rad3 shell_radial3(const DevShell& s, const double2* __restrict__ prims, double x,
+                                              double y, double z) {
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    Rad3 o{0.0, 0.0, 0.0};
+    if (r2 <= s.r2cut) {
+        for (int p = 0; p < s.n_prims; ++p) {
+            const double2 pr = prims[s.prim_start + p];
+            const double e = pr.y * exp(-pr.x * r2);
+            o.r0 += e;
+            o.r1 -= 2.0 * pr.x * e;
+            o.r2 += 4.0 * pr.x * pr.x * e;
+        }
+        if (fabs(o.r0) < 1e-18 && fabs(o.r1) < 1e-18 && fabs(o.r2) < 1e-18) o.r0 = o.r1 = o.r2 = 0.0;
+    }
+    return o;
+}
+
+// x^l and its first two derivatives, l <= 3 (pow_d0_d1_d2, ao_grid.rs:80-95).
+__device__ __forceinline__ void mono_d2(double x, int l, double& p, double& d, double& d2) {
+    const double x2 = x * x;
+    p = l == 0 ? 1.0 : (l == 1 ? x : (l == 2 ? x2 : x2 * x));
+    d = l == 0 ? 0.0 : (l == 1 ? 1.0 : (l == 2 ? 2.0 * x : 3.0 * x * x));
+    d2 = l <= 1 ? 0.0 : (l == 2 ? 2.0 : 6.0 * x);
+}
+
+// Value, gradient and Hessian (xx, xy, xz, yy, yz, zz) of one row.
+struct AoHess {
+    double v, g[3], h[6];
+};
+
+__device__ __forceinline__ AoHess ao_hess_row(const DevShell& s, const RowL& r, double x, double y, double z,
+                                              const Rad3& rr) {
+    AoHess o{};
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    double px, dpx, d2px, py, dpy, d2py, pz, dpz, d2pz;
+    mono_d2(dx, r.lmn & 3, px, dpx, d2px);
+    mono_d2(dy, (r.lmn >> 2) & 3, py, dpy, d2py);
+    mono_d2(dz, (r.lmn >> 4) & 3, pz, dpz, d2pz);
+    const double m = px * py * pz, sc = r.scale;
+    const double dm0 = dpx * py * pz, dm1 = px * dpy * pz, dm2 = px * py * dpz;
+    o.v = sc * m * rr.r0;
+    o.g[0] = sc * (dm0 * rr.r0 + m * dx * rr.r1);
+    o.g[1] = sc * (dm1 * rr.r0 + m * dy * rr.r1);
+    o.g[2] = sc * (dm2 * rr.r0 + m * dz * rr.r1);
+    o.h[0] = sc * (d2px * py * pz * rr.r0 + (2.0 * dm0 * dx + m) * rr.r1 + m * dx * dx * rr.r2);
+    o.h[1] = sc * (dpx * dpy * pz * rr.r0 + (dm0 * dy + dm1 * dx) * rr.r1 + m * dx * dy * rr.r2);
+    o.h[2] = sc * (dpx * py * dpz * rr.r0 + (dm0 * dz + dm2 * dx) * rr.r1 + m * dx * dz * rr.r2);
+    o.h[3] = sc * (px * d2py * pz * rr.r0 + (2.0 * dm1 * dy + m) * rr.r1 + m * dy * dy * rr.r2);
+    o.h[4] = sc * (px * dpy * dpz * rr.r0 + (dm1 * dz + dm2 * dy) * rr.r1 + m * dy * dz * rr.r2);
+    o.h[5] = sc * (px * py * d2pz * rr.r0 + (2.0 * dm2 * dz + m) * rr.r1 + m * dz * dz * rr.r2);
+    return o;
+}
+
 // Stage the rows, shells (with block-local primitive offsets) and primitives
 // of a block or group into LDS.  rows: (global AO, local shell); shl:
 // (global shell, local primitive offset).

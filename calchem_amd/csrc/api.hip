@@ -83,7 +83,9 @@ struct Plan {
     std::vector<MolInfo> mol_host;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, rho_tiles, groups, group_rows, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
-        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg;
+        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
+        asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
+    int n_atoms = 0;  // atoms set by calchem_hip_vxc_set_atoms (0 = not set)
     bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
@@ -101,7 +103,8 @@ struct Plan {
                              &block_sh, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
-                             &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg};
+                             &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
+                             &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
         return s;
@@ -523,6 +526,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         b = e;
     }
     P.n_groups = static_cast<int>(hg.size());
+    std::vector<int> hshatom(n_shells);
+    for (int s = 0; s < n_shells; ++s) hshatom[s] = sh[s].atom_index;
     std::vector<int> hgao(hgrows.size());
     for (size_t i = 0; i < hgrows.size(); ++i) hgao[i] = hgrows[i].x;
     P.n_pairs = static_cast<int>(hpairs.size());
@@ -573,7 +578,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
-         upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
+         upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
+         upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
@@ -1034,6 +1040,126 @@ int calchem_hip_vxc_graph_destroy(uint64_t handle, uint64_t graph) {
         P.graphs[graph - 1] = nullptr;
     }
     return kOk;
+}
+
+int calchem_hip_vxc_set_atoms(uint64_t handle, int n_atoms, const int* z, const double* xyz_bohr,
+                              const double* raw_weight, const int* host_atom) {
+    if (handle == 0 || n_atoms <= 0 || !z || !xyz_bohr || !raw_weight || !host_atom) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    if (P.n_mol != 1) return kUnsupported;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    if (n_atoms > 64 * 4) return kLimit;  // Becke derivative: up to 4 atoms per lane
+    std::vector<int> sa(P.n_shells);
+    if (hipMemcpy(sa.data(), P.shell_atom.p, sizeof(int) * sa.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return kLaunchFail;
+    for (int a : sa)
+        if (a < 0 || a >= n_atoms) return kMismatch;
+    std::vector<int> perm(P.n_pts), host(P.n_pts);
+    std::vector<double> raw(P.n_pts);
+    if (P.n_pts > 0 &&
+        hipMemcpy(perm.data(), P.perm.p, sizeof(int) * perm.size(), hipMemcpyDeviceToHost) != hipSuccess)
+        return kLaunchFail;
+    for (int i = 0; i < P.n_pts; ++i) {
+        const int c = perm[i];
+        if (host_atom[c] < 0 || host_atom[c] >= n_atoms || !std::isfinite(raw_weight[c])) return kBadArgs;
+        host[i] = host_atom[c];
+        raw[i] = raw_weight[c];
+    }
+    std::vector<double> inv((size_t)n_atoms * n_atoms), asy((size_t)n_atoms * n_atoms);
+    becke_pair_tables(n_atoms, z, xyz_bohr, inv.data(), asy.data());
+    std::vector<double> xyz(xyz_bohr, xyz_bohr + 3 * (size_t)n_atoms);
+    if (!upload(P.atom_xyz, xyz) || !upload(P.inv_r, inv) || !upload(P.asym, asy) || !upload(P.raw_s, raw) ||
+        !upload(P.host_s, host))
+        return kAllocFail;
+    P.n_atoms = n_atoms;
+    return kOk;
+}
+
+int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double* Pa, const double* Pb, double* grad) {
+    if (handle == 0 || !Pa || !grad) return kBadArgs;
+    if (xc < 0 || xc > 5) return kUnsupported;
+    if (!device_available()) return kNoDevice;
+    auto plan = find_plan(handle);
+    if (!plan) return kMismatch;
+    Plan& P = *plan;
+    if (P.n_mol != 1) return kUnsupported;
+    std::lock_guard<std::mutex> lk(P.mu);
+    hipSetDevice(P.device);
+    if (P.n_atoms <= 0 || !P.has_weights) return kMismatch;
+    const int na = P.n_atoms, width = 3 * na;
+    for (size_t c = 0; c < P.classes.size(); ++c)
+        if (grad_ao_lds_bytes(P.classes[c].max_mpad, P.classes[c].max_nsh, P.classes[c].max_prim) > P.lds_limit)
+            return kLimit;
+    const bool uks = Pb != nullptr;
+    const size_t nn = (size_t)P.n_ao * P.n_ao;
+    const long long n_tiles = (long long)P.n_rho_tiles, passes = uks ? 2 : 1;
+    const int pts_per_wave = 256;
+    const long long n_waves = (P.n_pts + pts_per_wave - 1) / pts_per_wave;
+    const long long rows = n_tiles * passes + n_waves;
+    const long long per_chunk = 512, chunks = (rows + per_chunk - 1) / per_chunk;
+    hipStream_t s = P.stream;
+    const size_t pb = sizeof(double4) * (size_t)std::max(P.n_pts, 1);
+    if (!P.gwx_a.p && (!P.gwx_a.alloc(pb) || !P.gwx_b.alloc(pb) ||
+                       !P.gfxc.alloc(sizeof(double) * (size_t)std::max(P.n_pts, 1))))
+        return kAllocFail;
+    if (!P.rho_sb.p) {
+        if (!P.rho_sb.alloc(pb) || hipMemset(P.rho_sb.p, 0, pb) != hipSuccess) return kAllocFail;
+    }
+    if (P.gpart.bytes < sizeof(double) * (size_t)rows * width && !P.gpart.alloc(sizeof(double) * (size_t)rows * width))
+        return kAllocFail;
+    if (P.gsum.bytes < sizeof(double) * (size_t)(chunks + 1) * width &&
+        !P.gsum.alloc(sizeof(double) * (size_t)(chunks + 1) * width))
+        return kAllocFail;
+    if (uks && !P.Pb_host.p && (!P.Pb_host.alloc(sizeof(double) * nn) || !P.Vb_host.alloc(sizeof(double) * nn)))
+        return kAllocFail;
+    if (hipMemcpyAsync(P.P64.p, Pa, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess) return kLaunchFail;
+    if (uks && hipMemcpyAsync(P.Pb_host.p, Pb, sizeof(double) * nn, hipMemcpyHostToDevice, s) != hipSuccess)
+        return kLaunchFail;
+    const double* dPa = P.P64.as<double>();
+    const double* dPb = uks ? P.Pb_host.as<double>() : nullptr;
+    // Densities at the points, then the functional.
+    int rc = run_rho(P, kRhoFp64, P.rho_s.as<double4>(), dPa, s, false);
+    if (rc == kOk && uks) rc = run_rho(P, kRhoFp64, P.rho_sb.as<double4>(), dPb, s, false);
+    if (rc != kOk) return rc;
+    XcGradArgs xa{P.rho_s.as<double4>(), uks ? P.rho_sb.as<double4>() : nullptr, P.pts.as<double4>(), P.n_pts, xc,
+                  P.gwx_a.as<double4>(), P.gwx_b.as<double4>(), P.gfxc.as<double>()};
+    if (launch_xc_grad(xa, s) != hipSuccess) return kLaunchFail;
+    // AO terms, one pass per spin (restricted: the total density once).
+    for (int pass = 0; pass < passes; ++pass) {
+        const double* dP = pass == 0 ? dPa : dPb;
+        GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP,
+                      P.psub64.as<double>(), nullptr, 0};
+        if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
+        for (size_t c = 0; c < P.classes.size(); ++c) {
+            const RhoClass& cl = P.classes[c];
+            GradAoArgs g{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
+                         P.rho_tiles.as<int2>() + cl.tile_begin, P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(),
+                         P.block_sh.as<int2>(), P.shell_atom.as<int>(), P.psub64.as<double>(),
+                         pass == 0 ? P.gwx_a.as<double4>() : P.gwx_b.as<double4>(), P.host_s.as<int>(), na, -2.0,
+                         P.gpart.as<double>() + (size_t)(pass * n_tiles + cl.tile_begin) * width};
+            if (launch_grad_ao(g, cl.n_tiles, grad_ao_lds_bytes(cl.max_mpad, cl.max_nsh, cl.max_prim), s) !=
+                hipSuccess)
+                return kLaunchFail;
+        }
+    }
+    // Becke weight derivatives.
+    GradBeckeArgs gb{P.pts.as<double4>(), P.raw_s.as<double>(), P.host_s.as<int>(), P.gfxc.as<double>(),
+                     P.atom_xyz.as<double>(), P.inv_r.as<double>(), P.asym.as<double>(), na, P.n_pts, pts_per_wave,
+                     P.gpart.as<double>() + (size_t)(passes * n_tiles) * width};
+    if (launch_grad_becke(gb, (int)n_waves, s) != hipSuccess) return kLaunchFail;
+    // Fixed-order sum of all partial rows.
+    if (launch_grad_sum(P.gpart.as<double>(), rows, width, per_chunk, P.gsum.as<double>(), s) != hipSuccess ||
+        launch_grad_sum(P.gsum.as<double>(), chunks, width, chunks, P.gsum.as<double>() + (size_t)chunks * width, s) !=
+            hipSuccess)
+        return kLaunchFail;
+    if (hipMemcpyAsync(grad, P.gsum.as<double>() + (size_t)chunks * width, sizeof(double) * width,
+                       hipMemcpyDeviceToHost, s) != hipSuccess)
+        return kLaunchFail;
+    return hipStreamSynchronize(s) == hipSuccess ? kOk : kLaunchFail;
 }
 
 int calchem_hip_vxc_set_profiling(uint64_t handle, int on) {

@@ -325,17 +325,7 @@ long build_grid_host(int n_atoms, const int* z, const double* xyz, int level, in
     }
     const long long n_raw = static_cast<long long>(raw.size());
     std::vector<double> inv_r((size_t)n_atoms * n_atoms, 0.0), aij((size_t)n_atoms * n_atoms, 0.0);
-    for (int i = 0; i < n_atoms; ++i)
-        for (int j = i + 1; j < n_atoms; ++j) {
-            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
-                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
-            const double rij = std::sqrt(dx * dx + dy * dy + dz * dz);
-            const double inv = rij >= 1e-14 ? 1.0 / rij : 0.0;
-            const double a = becke_a(z[i], z[j]);
-            inv_r[(size_t)i * n_atoms + j] = inv_r[(size_t)j * n_atoms + i] = inv;
-            aij[(size_t)i * n_atoms + j] = a;
-            aij[(size_t)j * n_atoms + i] = -a;
-        }
+    becke_pair_tables(n_atoms, z, xyz, inv_r.data(), aij.data());
     double *d_xyz = nullptr, *d_inv = nullptr, *d_a = nullptr, *d_w = nullptr;
     double4* d_raw = nullptr;
     int* d_host = nullptr;
@@ -383,6 +373,23 @@ long build_grid_host(int n_atoms, const int* z, const double* xyz, int level, in
         ++cnt;
     }
     return cnt;
+}
+
+void becke_pair_tables(int n_atoms, const int* z, const double* xyz, double* inv_r, double* aij) {
+    for (int i = 0; i < n_atoms; ++i) {
+        inv_r[(size_t)i * n_atoms + i] = 0.0;
+        aij[(size_t)i * n_atoms + i] = 0.0;
+        for (int j = i + 1; j < n_atoms; ++j) {
+            const double dx = xyz[3 * i] - xyz[3 * j], dy = xyz[3 * i + 1] - xyz[3 * j + 1],
+                         dz = xyz[3 * i + 2] - xyz[3 * j + 2];
+            const double rij = std::sqrt(dx * dx + dy * dy + dz * dz);
+            const double inv = rij >= 1e-14 ? 1.0 / rij : 0.0;
+            const double a = becke_a(z[i], z[j]);
+            inv_r[(size_t)i * n_atoms + j] = inv_r[(size_t)j * n_atoms + i] = inv;
+            aij[(size_t)i * n_atoms + j] = a;
+            aij[(size_t)j * n_atoms + i] = -a;
+        }
+    }
 }
 
 }  // namespace chx

@@ -47,6 +47,60 @@ struct XcUksArgs {
     double* n_part;
 };
 
+// XC nuclear gradient (grad_kernels.hip).
+struct XcGradArgs {
+    const double4* rho_a;   // per sorted point; total density when rho_b is null
+    const double4* rho_b;   // null: restricted (spin densities = half the total)
+    const double4* pts;
+    int n_pts;
+    int func;
+    double4* wx_a;          // (w v_rho_a, w u_a)
+    double4* wx_b;
+    double* fxc;            // f_xc per point (0 where screened)
+};
+
+struct GradAoArgs {
+    const DevShell* shells;
+    const double2* prims;
+    const DevAo* aos;
+    const double4* pts;
+    const int2* tiles;      // (block, first point within it)
+    const RhoBlock* blocks;
+    const int2* block_rows;
+    const int2* block_sh;
+    const int* shell_atom;  // atom of each global shell
+    const double* PS;       // gathered FP64 density blocks of this pass
+    const double4* wx;      // (w v_rho, w u) of this pass's spin, per sorted point
+    const int* host;        // host atom per sorted point
+    int n_atoms;
+    double factor;          // -2 (per spin; restricted: total density, once)
+    double* partial;        // [tile][3 n_atoms]
+};
+
+struct GradBeckeArgs {
+    const double4* pts;
+    const double* raw_w;    // per sorted point
+    const int* host;
+    const double* fxc;
+    const double* xyz;      // [3 n_atoms] bohr
+    const double* inv_r;    // [n_atoms^2] 1 / R_cd (0 for coincident atoms)
+    const double* asym;     // [n_atoms^2] a_cd
+    int n_atoms;
+    int n_pts;
+    int pts_per_wave;
+    double* partial;        // [wave][3 n_atoms]
+};
+
+size_t grad_ao_lds_bytes(int max_mpad, int max_nsh, int max_prim);
+hipError_t launch_xc_grad(const XcGradArgs& a, hipStream_t s);
+hipError_t launch_grad_ao(const GradAoArgs& a, int n_tiles, size_t lds, hipStream_t s);
+hipError_t launch_grad_becke(const GradBeckeArgs& a, int n_waves, hipStream_t s);
+hipError_t launch_grad_sum(const double* part, long long rows, int width, long long rows_per_chunk, double* out,
+                           hipStream_t s);
+// Becke pair tables of a molecule (grid.hip): 1 / R_cd and the size
+// adjustment a_cd (antisymmetric), row-major n x n.
+void becke_pair_tables(int n_atoms, const int* z, const double* xyz, double* inv_r, double* asym);
+
 // Launch class of rho tiles: blocks of similar size share one LDS budget.
 struct RhoClass {
     int tile_begin, n_tiles;  // range in the class-ordered tile list

@@ -118,6 +118,37 @@ class RksVxcEngine:
     def graph_destroy(self, graph: int) -> None:
         _native.check(self._lib.calchem_hip_vxc_graph_destroy(self.handle, graph), "calchem_hip_vxc_graph_destroy")
 
+    def set_atoms(self, mol, grid: MolecularGrid) -> None:
+        """Atoms (z, bohr) and the grid's raw weights / host atoms, for xc_gradient."""
+        z = np.ascontiguousarray(mol.z_array(), dtype=np.int32)
+        xyz = np.ascontiguousarray(mol.coords_bohr().ravel(), dtype=np.float64)
+        raw = np.ascontiguousarray(grid.points[:, 4], dtype=np.float64)
+        host = np.ascontiguousarray(grid.points[:, 5].astype(np.int32))
+        if raw.shape[0] != self.n_points:
+            raise ValueError("grid does not match the engine's points")
+        _native.check(self._lib.calchem_hip_vxc_set_atoms(self.handle, len(z), _native.ptr(z), _native.ptr(xyz),
+                                                          _native.ptr(raw), _native.ptr(host)),
+                      "calchem_hip_vxc_set_atoms")
+        self._n_atoms = len(z)
+
+    def xc_gradient(self, func: str, P: np.ndarray, Pb: Optional[np.ndarray] = None) -> np.ndarray:
+        """dE_xc/dR_A [n_atoms, 3] at fixed density (xc_nuclear_gradient_with_gpu,
+        vxc.rs:2030-2334): restricted for the total density P, or unrestricted for
+        spin densities (P, Pb)."""
+        n = self.n_ao
+        Pa = np.ascontiguousarray(P, dtype=np.float64)
+        if Pa.shape != (n, n) or (Pb is not None and np.shape(Pb) != (n, n)):
+            raise ValueError("density shape does not match the basis")
+        Pb_c = None if Pb is None else np.ascontiguousarray(Pb, dtype=np.float64)
+        na = getattr(self, "_n_atoms", 0)
+        if na <= 0:
+            raise _native.CalchemHipError("calchem_hip_vxc_xc_gradient (set_atoms first)", -2)
+        g = np.zeros(3 * na)
+        rc = self._lib.calchem_hip_vxc_xc_gradient(self.handle, xc_id(func), _native.ptr(Pa),
+                                                   None if Pb_c is None else _native.ptr(Pb_c), _native.ptr(g))
+        _native.check(rc, "calchem_hip_vxc_xc_gradient")
+        return g.reshape(na, 3)
+
     def set_profiling(self, on: bool) -> None:
         _native.check(self._lib.calchem_hip_vxc_set_profiling(self.handle, 1 if on else 0),
                       "calchem_hip_vxc_set_profiling")

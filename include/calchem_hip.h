@@ -179,6 +179,21 @@ int calchem_hip_vxc_graph_create(uint64_t handle, int xc, int precision, const d
 int calchem_hip_vxc_graph_launch(uint64_t handle, uint64_t graph, void *stream);
 int calchem_hip_vxc_graph_destroy(uint64_t handle, uint64_t graph);
 
+/* ---- XC nuclear gradient ----
+ * Replaces xc_nuclear_gradient_with_gpu (crates/calchem-dft/src/vxc.rs:2030-2334;
+ * per-point semantics accumulate_xc_grad_points, vxc.rs:1769-1899, and
+ * becke_weight_nuclear_gradient, crates/calchem-dft/src/grid.rs:423-584).
+ * set_atoms gives a single-molecule plan its atoms (z, bohr; the shells'
+ * atom_index name them) and, per point in the order handed to
+ * calchem_hip_vxc_create, the raw (pre-Becke) weight and host atom
+ * (MolecularGrid.points raw_weight / host_atom); up to 256 atoms.
+ * xc_gradient returns grad[n_atoms*3] = dE_xc/dR_A at fixed density matrix:
+ * unrestricted for spin densities Pa, Pb, or restricted with Pb = NULL and Pa
+ * the total density. */
+int calchem_hip_vxc_set_atoms(uint64_t handle, int n_atoms, const int *z, const double *xyz_bohr,
+                              const double *raw_weight, const int *host_atom);
+int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double *Pa, const double *Pb, double *grad);
+
 /* ---- Fused build (functional on the device) ----
  * Replaces the per-geometry setup of RksVxcEngine::new_with_gpu
  * (crates/calchem-dft/src/vxc.rs:429-595): xyzw[n_pts*4] = (x, y, z, weight)

@@ -195,6 +195,18 @@ static inline void powd(double x, int l, double *p, double *d) {
     }
 }
 
+/* pow_d0_d1_d2 (ao_grid.rs:80-95): x^l and its first two derivatives. */
+static inline void pow012(double x, int l, double *p, double *d, double *d2) {
+    switch (l) {
+    case 0: *p = 1.0; *d = 0.0; *d2 = 0.0; break;
+    case 1: *p = x; *d = 1.0; *d2 = 0.0; break;
+    case 2: *p = x * x; *d = 2.0 * x; *d2 = 2.0; break;
+    case 3: *p = x * x * x; *d = 3.0 * x * x; *d2 = 6.0 * x; break;
+    default:
+        *p = ipow(x, l); *d = (double)l * ipow(x, l - 1); *d2 = (double)l * (l - 1) * ipow(x, l - 2);
+    }
+}
+
 /* One AO in the active list of a point. */
 typedef struct { int mu; double val, g[3]; } aoval_t;
 
@@ -1405,6 +1417,256 @@ int or_vxc_uks_perpoint(const or_basis_t *b, long n_pts, const double *pts, int 
     *exc = e;
     if (nelec) *nelec = ne;
     free(X.vloc); free(X.vlocb); free(X.e); free(X.ne);
+    shells_free(&S);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* XC nuclear gradient (xc_nuclear_gradient_cpu / accumulate_xc_grad_points, */
+/* vxc.rs:1769-1899; becke_weight_nuclear_gradient, grid.rs:423-584).        */
+/* ------------------------------------------------------------------------ */
+typedef struct { int mu, atom; double val, g[3], h[6]; } aohess_t; /* h: xx xy xz yy yz zz */
+
+/* eval_aos_hess (ao_grid.rs:234-299): values, gradients and Hessians of every
+ * AO of every shell within its cutoff; no per-AO drop rule.  shell_atom maps
+ * shells to atoms. */
+static int eval_aos_hess(const shells_t *S, const int *shell_atom, const double r[3], aohess_t *out) {
+    int n = 0;
+    for (int k = 0; k < S->n_shells; ++k) {
+        const shell_t *sh = &S->sh[k];
+        double dx = r[0] - sh->center[0], dy = r[1] - sh->center[1], dz = r[2] - sh->center[2];
+        double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 > sh->r2_cutoff) continue;
+        double r0 = 0.0, r1 = 0.0, rr2 = 0.0;
+        for (int p = 0; p < sh->n_prim; ++p) {
+            double e = sh->wnorm[p] * exp(-sh->alpha[p] * r2);
+            r0 += e;
+            r1 -= 2.0 * sh->alpha[p] * e;
+            rr2 += 4.0 * sh->alpha[p] * sh->alpha[p] * e;
+        }
+        if (fabs(r0) < 1e-18 && fabs(r1) < 1e-18 && fabs(rr2) < 1e-18) continue;
+        for (int c = 0; c < sh->nfn; ++c) {
+            const int *lmn = comp_of(sh->l, c);
+            double px, dpx, d2px, py, dpy, d2py, pz, dpz, d2pz;
+            pow012(dx, lmn[0], &px, &dpx, &d2px);
+            pow012(dy, lmn[1], &py, &dpy, &d2py);
+            pow012(dz, lmn[2], &pz, &dpz, &d2pz);
+            double m = px * py * pz, sc = sh->scale[c];
+            double dm[3] = {dpx * py * pz, px * dpy * pz, px * py * dpz};
+            aohess_t *o = &out[n++];
+            o->mu = sh->ao_offset + c;
+            o->atom = shell_atom[k];
+            o->val = sc * m * r0;
+            o->g[0] = sc * (dm[0] * r0 + m * dx * r1);
+            o->g[1] = sc * (dm[1] * r0 + m * dy * r1);
+            o->g[2] = sc * (dm[2] * r0 + m * dz * r1);
+            o->h[0] = sc * (d2px * py * pz * r0 + (2.0 * dm[0] * dx + m) * r1 + m * dx * dx * rr2);
+            o->h[1] = sc * (dpx * dpy * pz * r0 + (dm[0] * dy + dm[1] * dx) * r1 + m * dx * dy * rr2);
+            o->h[2] = sc * (dpx * py * dpz * r0 + (dm[0] * dz + dm[2] * dx) * r1 + m * dx * dz * rr2);
+            o->h[3] = sc * (px * d2py * pz * r0 + (2.0 * dm[1] * dy + m) * r1 + m * dy * dy * rr2);
+            o->h[4] = sc * (px * dpy * dpz * r0 + (dm[1] * dz + dm[2] * dy) * r1 + m * dy * dz * rr2);
+            o->h[5] = sc * (px * py * d2pz * r0 + (2.0 * dm[2] * dz + m) * r1 + m * dz * dz * rr2);
+        }
+    }
+    return n;
+}
+
+/* becke_step_and_deriv (grid.rs:129-140): s(nu) and ds/dnu. */
+static void becke_step_deriv(double nu, double *s, double *ds) {
+    double x = clamp1(nu);
+    double f1 = 0.5 * x * (3.0 - x * x), d1 = 1.5 * (1.0 - x * x);
+    double f2 = 0.5 * f1 * (3.0 - f1 * f1), d2 = 1.5 * (1.0 - f1 * f1);
+    double f3 = 0.5 * f2 * (3.0 - f2 * f2), d3 = 1.5 * (1.0 - f2 * f2);
+    *s = 0.5 * (1.0 - f3);
+    *ds = -0.5 * d3 * d2 * d1;
+}
+
+typedef struct {
+    int n;
+    const int *z;
+    const double *xyz;
+    double *vra, *dist, *smat, *dsdmu, *cellp; /* scratch per thread */
+} bgrad_t;
+
+/* becke_weight_nuclear_gradient (grid.rs:423-584): adds raw_w f_xc dw_host/dR_A. */
+static void becke_grad(bgrad_t *B, int host, const double r[3], double raw_w, double fxc, double *grad) {
+    int n = B->n;
+    if (n <= 1 || fabs(fxc) < 1e-18 || raw_w == 0.0) return;
+    for (int c = 0; c < n; ++c) {
+        double dx = r[0] - B->xyz[3 * c], dy = r[1] - B->xyz[3 * c + 1], dz = r[2] - B->xyz[3 * c + 2];
+        B->vra[3 * c] = dx; B->vra[3 * c + 1] = dy; B->vra[3 * c + 2] = dz;
+        B->dist[c] = sqrt(dx * dx + dy * dy + dz * dz);
+        B->cellp[c] = 1.0;
+    }
+    for (long k = 0; k < (long)n * n; ++k) { B->smat[k] = 1.0; B->dsdmu[k] = 0.0; }
+    for (int c = 0; c < n; ++c)
+        for (int d = c + 1; d < n; ++d) {
+            const double *ac = B->xyz + 3 * c, *ad = B->xyz + 3 * d;
+            double rcd = sqrt((ac[0] - ad[0]) * (ac[0] - ad[0]) + (ac[1] - ad[1]) * (ac[1] - ad[1]) +
+                              (ac[2] - ad[2]) * (ac[2] - ad[2]));
+            if (rcd < 1e-14) continue;
+            double mu = clamp1((B->dist[c] - B->dist[d]) / rcd);
+            double a = becke_asym(B->z[c], B->z[d]);
+            double nu = mu + a * (1.0 - mu * mu), scd, dsdnu;
+            becke_step_deriv(nu, &scd, &dsdnu);
+            B->smat[c * n + d] = scd;
+            B->smat[d * n + c] = 1.0 - scd;
+            B->dsdmu[c * n + d] = B->dsdmu[d * n + c] = dsdnu * (1.0 - 2.0 * a * mu);
+            B->cellp[c] *= scd;
+            B->cellp[d] *= 1.0 - scd;
+        }
+    double zs = 0.0;
+    for (int c = 0; c < n; ++c) zs += B->cellp[c];
+    if (zs <= 1e-30) return;
+    double wn = B->cellp[host] / zs, pref = raw_w * fxc / zs;
+    double hacc[3] = {0, 0, 0};
+    for (int a = 0; a < n; ++a) {
+        if (a == host) continue;
+        double ra = B->dist[a];
+        if (ra < 1e-14) continue;
+        const double *pa = B->xyz + 3 * a;
+        double dra[3] = {-B->vra[3 * a] / ra, -B->vra[3 * a + 1] / ra, -B->vra[3 * a + 2] / ra};
+        double dph[3] = {0, 0, 0}, dzs[3] = {0, 0, 0};
+        for (int b = 0; b < n; ++b) {
+            if (b == a) continue;
+            const double *pb = B->xyz + 3 * b;
+            double rv[3] = {pa[0] - pb[0], pa[1] - pb[1], pa[2] - pb[2]};
+            double rab = sqrt(rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2]);
+            if (rab < 1e-14) continue;
+            double mu = clamp1((ra - B->dist[b]) / rab), inv = 1.0 / rab;
+            double dmu[3];
+            for (int q = 0; q < 3; ++q) dmu[q] = (dra[q] - mu * rv[q] * inv) * inv;
+            double slope = B->dsdmu[a * n + b];
+            if (fabs(slope) < 1e-30) continue;
+            double pab, pba;
+            { /* prod over d != a, skip b of s_ad (and b's row skipping a) */
+                double sv = B->smat[a * n + b];
+                if (sv > 1e-12) pab = B->cellp[a] / sv;
+                else { pab = 1.0; for (int d = 0; d < n; ++d) if (d != a && d != b) pab *= B->smat[a * n + d]; }
+                sv = B->smat[b * n + a];
+                if (sv > 1e-12) pba = B->cellp[b] / sv;
+                else { pba = 1.0; for (int d = 0; d < n; ++d) if (d != b && d != a) pba *= B->smat[b * n + d]; }
+            }
+            for (int q = 0; q < 3; ++q) {
+                double dsab = slope * dmu[q];
+                double ta = pab * dsab, tb = -pba * dsab;
+                dzs[q] += ta + tb;
+                if (b == host) dph[q] += tb;
+            }
+        }
+        for (int q = 0; q < 3; ++q) {
+            double dwf = pref * (dph[q] - wn * dzs[q]);
+            grad[3 * a + q] += dwf;
+            hacc[q] -= dwf;
+        }
+    }
+    for (int q = 0; q < 3; ++q) grad[3 * host + q] += hacc[q];
+}
+
+typedef struct {
+    const shells_t *S;
+    const int *shell_atom;
+    const double *pts6, *Pa, *Pb;
+    long n_pts;
+    int func, n, n_atoms;
+    const int *z;
+    const double *xyz;
+    double *gpart; /* [nt][n_atoms*3] */
+} grad_ctx_t;
+
+static void grad_worker(void *vctx, long cb, long ce, int tid) {
+    grad_ctx_t *X = (grad_ctx_t *)vctx;
+    const int n = X->n, na = X->n_atoms;
+    long g0 = cb * BATCH, g1 = ce * BATCH < X->n_pts ? ce * BATCH : X->n_pts;
+    double *grad = X->gpart + (size_t)tid * na * 3;
+    aohess_t *act = (aohess_t *)malloc(sizeof(aohess_t) * (size_t)(n + 1));
+    double *sa = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *sb = (double *)malloc(sizeof(double) * (size_t)(n + 1));
+    double *gpa = (double *)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    double *gpb = (double *)malloc(sizeof(double) * 3 * (size_t)(n + 1));
+    bgrad_t B;
+    B.n = na; B.z = X->z; B.xyz = X->xyz;
+    B.vra = (double *)malloc(sizeof(double) * 3 * (size_t)na);
+    B.dist = (double *)malloc(sizeof(double) * (size_t)na);
+    B.cellp = (double *)malloc(sizeof(double) * (size_t)na);
+    B.smat = (double *)malloc(sizeof(double) * (size_t)na * na);
+    B.dsdmu = (double *)malloc(sizeof(double) * (size_t)na * na);
+    static const int hq[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}}; /* hess_row(q) */
+    for (long g = g0; g < g1; ++g) {
+        const double *p = X->pts6 + 6 * g;
+        const double w = p[3], raw = p[4];
+        const int host = (int)p[5];
+        if (fabs(raw) < 1e-24) continue;
+        int na_ = eval_aos_hess(X->S, X->shell_atom, p, act);
+        if (na_ == 0) continue;
+        double rha = 0, rhb = 0, ga[3] = {0, 0, 0}, gb[3] = {0, 0, 0};
+        for (int i = 0; i < na_; ++i) {
+            const double *ra_ = X->Pa + (size_t)act[i].mu * n, *rb_ = X->Pb + (size_t)act[i].mu * n;
+            double s_a = 0, s_b = 0, va[3] = {0, 0, 0}, vb[3] = {0, 0, 0};
+            for (int j = 0; j < na_; ++j) {
+                double pa_ = ra_[act[j].mu], pb_ = rb_[act[j].mu];
+                s_a += pa_ * act[j].val;
+                s_b += pb_ * act[j].val;
+                for (int q = 0; q < 3; ++q) { va[q] += pa_ * act[j].g[q]; vb[q] += pb_ * act[j].g[q]; }
+            }
+            sa[i] = s_a; sb[i] = s_b;
+            for (int q = 0; q < 3; ++q) { gpa[3 * i + q] = va[q]; gpb[3 * i + q] = vb[q]; }
+            rha += act[i].val * s_a;
+            rhb += act[i].val * s_b;
+            for (int q = 0; q < 3; ++q) { ga[q] += 2.0 * act[i].g[q] * s_a; gb[q] += 2.0 * act[i].g[q] * s_b; }
+        }
+        double ra = rha > 0 ? rha : 0, rb = rhb > 0 ? rhb : 0;
+        if (ra + rb < DENSITY_SCREEN) continue;
+        double gaa = ga[0] * ga[0] + ga[1] * ga[1] + ga[2] * ga[2];
+        double gab = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
+        double gbb = gb[0] * gb[0] + gb[1] * gb[1] + gb[2] * gb[2];
+        xc_t o = eval_xc(X->func, ra, rb, gaa, gab, gbb);
+        if (fabs(w) > 1e-24) {
+            double ua[3], ub[3], sum[3] = {0, 0, 0};
+            for (int q = 0; q < 3; ++q) {
+                ua[q] = 2.0 * o.gaa * ga[q] + o.gab * gb[q];
+                ub[q] = 2.0 * o.gbb * gb[q] + o.gab * ga[q];
+            }
+            for (int i = 0; i < na_; ++i) {
+                const double *G = act[i].g, *H = act[i].h;
+                double uga = ua[0] * gpa[3 * i] + ua[1] * gpa[3 * i + 1] + ua[2] * gpa[3 * i + 2];
+                double ugb = ub[0] * gpb[3 * i] + ub[1] * gpb[3 * i + 1] + ub[2] * gpb[3 * i + 2];
+                for (int q = 0; q < 3; ++q) {
+                    double hua = ua[0] * H[hq[q][0]] + ua[1] * H[hq[q][1]] + ua[2] * H[hq[q][2]];
+                    double hub = ub[0] * H[hq[q][0]] + ub[1] * H[hq[q][1]] + ub[2] * H[hq[q][2]];
+                    double ta = o.va * G[q] * sa[i] + hua * sa[i] + G[q] * uga;
+                    double tb = o.vb * G[q] * sb[i] + hub * sb[i] + G[q] * ugb;
+                    double f = -2.0 * w * (ta + tb);
+                    grad[3 * act[i].atom + q] += f;
+                    sum[q] += f;
+                }
+            }
+            for (int q = 0; q < 3; ++q) grad[3 * host + q] -= sum[q];
+        }
+        becke_grad(&B, host, p, raw, o.f, grad);
+    }
+    free(act); free(sa); free(sb); free(gpa); free(gpb);
+    free(B.vra); free(B.dist); free(B.cellp); free(B.smat); free(B.dsdmu);
+}
+
+int or_xc_gradient(const or_basis_t *b, int n_atoms, const int *z, const double *xyz_bohr, long n_pts,
+                   const double *pts6, int func, const double *Pa, const double *Pb, int nt, double *grad) {
+    if (!b || n_atoms <= 0 || !z || !xyz_bohr || !pts6 || !Pa || !Pb || !grad) return -1;
+    for (int k = 0; k < 3 * n_atoms; ++k) grad[k] = 0.0;
+    if (n_atoms <= 1 || n_pts <= 0) return 0;
+    shells_t S;
+    if (shells_build(b, &S) != 0) return -1;
+    if (nt < 1) nt = 1;
+    long nb = (n_pts + BATCH - 1) / BATCH;
+    if (nt > nb) nt = (int)nb;
+    grad_ctx_t X;
+    X.S = &S; X.shell_atom = b->atom; X.pts6 = pts6; X.Pa = Pa; X.Pb = Pb; X.n_pts = n_pts;
+    X.func = func; X.n = S.n_ao; X.n_atoms = n_atoms; X.z = z; X.xyz = xyz_bohr;
+    X.gpart = (double *)calloc((size_t)nt * n_atoms * 3, sizeof(double));
+    parallel_for(nb, nt, grad_worker, &X);
+    for (int t = 0; t < nt; ++t)
+        for (int k = 0; k < 3 * n_atoms; ++k) grad[k] += X.gpart[(size_t)t * n_atoms * 3 + k];
+    free(X.gpart);
     shells_free(&S);
     return 0;
 }

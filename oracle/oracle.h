@@ -74,6 +74,14 @@ long or_vxc_rks_cached_range(const or_basis_t *b, long n_pts, const double *pts,
                              const double *P, int n_threads, long batch_begin, long batch_end,
                              double *V, double *exc, double *nelec, double *flops);
 
+/* XC nuclear gradient dE_xc/dR_A (xc_nuclear_gradient_cpu semantics:
+ * accumulate_xc_grad_points, vxc.rs:1769-1899, and
+ * becke_weight_nuclear_gradient, grid.rs:423-584) for spin densities Pa, Pb
+ * (RKS: Pa = Pb = P/2).  pts6: [n_pts*6] grid rows (x, y, z, w, raw_w,
+ * host).  The basis' shell atom indices name the atoms.  grad: [n_atoms*3]. */
+int or_xc_gradient(const or_basis_t *b, int n_atoms, const int *z, const double *xyz_bohr, long n_pts,
+                   const double *pts6, int func, const double *Pa, const double *Pb, int nt, double *grad);
+
 /* Per-point UKS path (vxc.rs:1141-1470). */
 int or_vxc_uks_perpoint(const or_basis_t *b, long n_pts, const double *pts, int func,
                         const double *Pa, const double *Pb, int n_threads, double *Va,

@@ -60,6 +60,8 @@ def lib():
                                           C.c_int, _dp, _dp, C.POINTER(C.c_double),
                                           C.POINTER(C.c_double)]
         L.or_overlap.argtypes = [C.POINTER(_Basis), _dp]
+        L.or_xc_gradient.argtypes = [C.POINTER(_Basis), C.c_int, _ip, _dp, C.c_long, _dp, C.c_int, _dp, _dp,
+                                     C.c_int, _dp]
         L.or_rho_points.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, C.c_int, _dp]
         L.or_ao_dense.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, _dp]
         _lib = L
@@ -183,6 +185,21 @@ def vxc_uks(ob: OracleBasis, grid_rows, func, Pa, Pb, threads=None):
     if rc != 0:
         raise RuntimeError(rc)
     return e.value, Va.reshape(n, n), Vb.reshape(n, n), ne.value
+
+
+def xc_gradient(ob: OracleBasis, mol, grid_rows, func, Pa, Pb, threads=None):
+    """dE_xc/dR_A, [n_atoms, 3], for spin densities Pa, Pb (RKS: Pa = Pb = P/2)."""
+    z = np.ascontiguousarray(mol.z_array(), dtype=np.int32)
+    xyz = np.ascontiguousarray(mol.coords_bohr().ravel())
+    pts = np.ascontiguousarray(np.asarray(grid_rows)[:, :6])
+    g = np.zeros(3 * len(z))
+    nt = threads or min(16, os.cpu_count() or 1)
+    rc = lib().or_xc_gradient(C.byref(ob.struct), len(z), z, xyz, pts.shape[0], pts.ravel(), FUNC_IDS[func],
+                              np.ascontiguousarray(Pa, dtype=np.float64).ravel(),
+                              np.ascontiguousarray(Pb, dtype=np.float64).ravel(), nt, g)
+    if rc != 0:
+        raise RuntimeError(rc)
+    return g.reshape(-1, 3)
 
 
 def overlap(ob: OracleBasis):
