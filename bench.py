@@ -319,6 +319,15 @@ def main() -> int:
         log(f"{prec}: {ms:.3f} ms/build ({1000.0 / ms:.2f} builds/s), E_xc={en[0]:.12f} N={en[1]:.10f}, "
             f"kernels {kms}")
 
+    # XC nuclear gradient of the same workload (restricted, FP64), host buffers.
+    eng.set_atoms(mol, grid)
+    eng.xc_gradient(func, P)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        G = eng.xc_gradient(func, P)
+    grad_ms = (time.perf_counter() - t0) / 3 * 1e3
+    log(f"XC nuclear gradient: {grad_ms:.2f} ms, |sum over atoms| {np.abs(G.sum(axis=0)).max():.1e}")
+
     # Host-buffer path (P copied in, V copied out over PCIe, synchronous), for reference only.
     pcie = {}
     for prec in ("fp64", "fp32"):
@@ -394,6 +403,7 @@ def main() -> int:
         "fp32_builds_per_s": 1000.0 / r32["ms"],
         "mixed_vs_fp64_speedup": r64["ms"] / r32["ms"],
         "pcie_inclusive_builds_per_s": pcie,
+        "xc_gradient_ms": grad_ms,
         "mixed_errors_vs_fp64_gpu": {"dE_xc": dE, "max_dV": dV32, "dN": dN, "gate_1e-7": gate},
         "fp64_gpu_vs_oracle_sample": parity,
         "build_tflops": build_tflops,

@@ -225,7 +225,11 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = a.n_atoms, lane = threadIdx.x;
     double* sDist = reinterpret_cast<double*>(smem);  // [n]
-    double* sCell = sDist + n;                        // [n]
+    double* sCell = sDist + n;                        // [n] P_c = prod_{d != c} s_cd
+    double* sBig = sCell + n;                         // [n] product of the factors s_cd > 1e-12
+    double* sTiny = sBig + n;                         // [n] product of the nonzero factors s_cd <= 1e-12
+    int* sNTiny = reinterpret_cast<int*>(sTiny + n);  // [n] how many factors are <= 1e-12
+    int* sNZero = sNTiny + n;                         // [n] how many factors are exactly 0
     double acc[kBeckeAtomsPerLane][3];
 #pragma unroll
     for (int k = 0; k < kBeckeAtomsPerLane; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
@@ -241,20 +245,38 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
             sDist[c] = sqrt(dx * dx + dy * dy + dz * dz);
         }
         __syncthreads();
-        // cell function P_c = prod_{d != c} s_cd
+        // cell function P_c = prod_{d != c} s_cd, and the same product split
+        // into its factors above and at or below 1e-12 (for the products that
+        // leave one factor out)
         for (int c = lane; c < n; c += 64) {
-            double pc = 1.0;
+            double pc = 1.0, big = 1.0, tiny = 1.0;
+            int nt = 0, nz = 0;
             for (int d = 0; d < n; ++d) {
                 if (d == c) continue;
-                const double inv = a.inv_r[c * n + d];
+                // inv_r is symmetric and asym antisymmetric: the transposed
+                // reads [d][c] are contiguous over the lanes
+                const double inv = a.inv_r[d * n + c];
                 if (inv == 0.0) continue;
                 const double mu = fmin(fmax((sDist[c] - sDist[d]) * inv, -1.0), 1.0);
-                const double nu = mu + a.asym[c * n + d] * (1.0 - mu * mu);
+                const double nu = mu - a.asym[d * n + c] * (1.0 - mu * mu);
                 double s, ds;
                 becke_step_deriv(nu, s, ds);
                 pc *= s;
+                if (s > 1e-12) {
+                    big *= s;
+                } else {
+                    ++nt;
+                    if (s > 0.0)
+                        tiny *= s;
+                    else
+                        ++nz;
+                }
             }
             sCell[c] = pc;
+            sBig[c] = big;
+            sTiny[c] = tiny;
+            sNTiny[c] = nt;
+            sNZero[c] = nz;
         }
         __syncthreads();
         double zs = 0.0;
@@ -273,10 +295,10 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                 double dph[3] = {0.0, 0.0, 0.0}, dzs[3] = {0.0, 0.0, 0.0};
                 for (int b = 0; b < n; ++b) {
                     if (b == at) continue;
-                    const double inv = a.inv_r[at * n + b];
+                    const double inv = a.inv_r[b * n + at];
                     if (inv == 0.0) continue;
                     const double mu = fmin(fmax((ra - sDist[b]) * inv, -1.0), 1.0);
-                    const double asy = a.asym[at * n + b];
+                    const double asy = -a.asym[b * n + at];
                     const double nu = mu + asy * (1.0 - mu * mu);
                     double s_ab, dsdnu;
                     becke_step_deriv(nu, s_ab, dsdnu);
@@ -284,33 +306,22 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                     if (fabs(slope) < 1e-30) continue;
                     const double rv[3] = {a.xyz[3 * at] - a.xyz[3 * b], a.xyz[3 * at + 1] - a.xyz[3 * b + 1],
                                           a.xyz[3 * at + 2] - a.xyz[3 * b + 2]};
-                    // products over the other pairs of at and of b
-                    double p_ab, p_ba;
-                    if (s_ab > 1e-12) {
-                        p_ab = sCell[at] / s_ab;
-                    } else {
-                        p_ab = 1.0;
-                        for (int d = 0; d < n; ++d) {
-                            if (d == at || d == b || a.inv_r[at * n + d] == 0.0) continue;
-                            const double m2 = fmin(fmax((ra - sDist[d]) * a.inv_r[at * n + d], -1.0), 1.0);
-                            double sd, dd;
-                            becke_step_deriv(m2 + a.asym[at * n + d] * (1.0 - m2 * m2), sd, dd);
-                            p_ab *= sd;
-                        }
-                    }
-                    const double s_ba = 1.0 - s_ab;
-                    if (s_ba > 1e-12) {
-                        p_ba = sCell[b] / s_ba;
-                    } else {
-                        p_ba = 1.0;
-                        for (int d = 0; d < n; ++d) {
-                            if (d == b || d == at || a.inv_r[b * n + d] == 0.0) continue;
-                            const double m2 = fmin(fmax((sDist[b] - sDist[d]) * a.inv_r[b * n + d], -1.0), 1.0);
-                            double sd, dd;
-                            becke_step_deriv(m2 + a.asym[b * n + d] * (1.0 - m2 * m2), sd, dd);
-                            p_ba *= sd;
-                        }
-                    }
+                    // s_ba exactly as the cell loop formed it for atom b, so that it
+                    // is one of the factors counted there
+                    const double mu_b = fmin(fmax((sDist[b] - ra) * inv, -1.0), 1.0);
+                    double s_ba, ds_unused;
+                    becke_step_deriv(mu_b - asy * (1.0 - mu_b * mu_b), s_ba, ds_unused);
+                    // products over the other pairs of at and of b: P / s when s is
+                    // above 1e-12; else from the split products, in which s is one
+                    // of the factors at or below 1e-12 (and one of the zeros if 0)
+                    auto except = [&](int c, double sv) {
+                        if (sv > 1e-12) return sCell[c] / sv;
+                        if (sNTiny[c] == 1) return sBig[c];
+                        if (sv > 0.0) return sNZero[c] > 0 ? 0.0 : sBig[c] * (sTiny[c] / sv);
+                        return sNZero[c] > 1 ? 0.0 : sBig[c] * sTiny[c];
+                    };
+                    const double p_ab = except(at, s_ab);
+                    const double p_ba = except(b, s_ba);
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
                         const double dsab = slope * (dra[c] - mu * rv[c] * inv) * inv;
@@ -391,7 +402,8 @@ hipError_t launch_grad_ao(const GradAoArgs& a, int n_tiles, size_t lds, hipStrea
 
 hipError_t launch_grad_becke(const GradBeckeArgs& a, int n_waves, hipStream_t s) {
     if (n_waves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grad_becke, dim3(n_waves), dim3(64), sizeof(double) * 2 * (size_t)a.n_atoms, s, a);
+    hipLaunchKernelGGL(k_grad_becke, dim3(n_waves), dim3(64), (sizeof(double) * 4 + sizeof(int) * 2) * (size_t)a.n_atoms,
+                       s, a);
     return hipGetLastError();
 }
 
