@@ -1,7 +1,7 @@
 // vxc_kernels.hip — gfx950 kernels of the RKS Vxc / E_xc / N build.
 //
 // One build (crates/calchem-dft/src/vxc.rs:801-985 restated for CDNA4):
-//   k_rho     per (block, 32 or 16 points): the block's shells, primitives and
+//   k_rho     per (block, 16 points): the block's shells, primitives and
 //             AO rows staged in LDS, radial sums per (shell, point), AO values
 //             into LDS, T = P_sub Phi on MFMA (P_sub rows read from L2),
 //             rho = sum_i Phi_i T_i and grad rho = 2 sum_i dPhi_i T_i.
