@@ -81,7 +81,8 @@ struct Plan {
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
-    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, rho_tiles, groups, group_rows, group_ao,
+    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, block_meta, rho_tiles, groups, group_rows,
+        group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
         asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
@@ -100,7 +101,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
+                             &block_sh, &block_meta, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
@@ -170,6 +171,15 @@ bool upload(DevBuf& d, const std::vector<T>& h) {
 }
 
 int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Appends n records of a type whose size is a multiple of 16 bytes.
+template <typename T>
+void push_records(std::vector<uint4>& out, const T* rec, int n) {
+    static_assert(sizeof(T) % sizeof(uint4) == 0, "records are whole 16-byte units");
+    const size_t at = out.size();
+    out.resize(at + sizeof(T) / sizeof(uint4) * (size_t)n);
+    if (n > 0) std::memcpy(out.data() + at, rec, sizeof(T) * (size_t)n);
+}
 
 // One molecule of a plan as the caller hands it in: its shells
 // [shell_begin, shell_end) (ao_offset local to the molecule), its points
@@ -383,6 +393,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     // with max |phi| above the mixed threshold first (each part ascending);
     // each row names its shell in the block's shell list.
     std::vector<RhoBlock> hb(P.n_blocks);
+    std::vector<uint4> hmeta;
     std::vector<int2> hrows;
     std::vector<int2> hsh;  // (global shell, block-local primitive offset)
     std::vector<std::vector<int>> blist(P.n_blocks);
@@ -417,6 +428,28 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         }
         B.n_sh = static_cast<int>(hsh.size()) - B.sh_begin;
         hrows.resize(hrows.size() + (mp - m), make_int2(-1, 0));
+        // Packed records, the layout the rho kernel copies into LDS as is.
+        B.meta = static_cast<long long>(hmeta.size());
+        for (int r = 0; r < mp; ++r) {
+            RowL o{0.0, 0, -1};
+            if (r < m) {
+                const int2 e = hrows[B.ao_begin + r];
+                const DevAo& ao = hao[e.x];
+                o.scale = ao.scale;
+                o.lmn = ao.lx | (ao.ly << 2) | (ao.lz << 4);
+                o.sh = e.y;
+            }
+            push_records(hmeta, &o, 1);
+        }
+        for (int k = 0; k < B.n_sh; ++k) {
+            DevShell d = hs[hsh[B.sh_begin + k].x];
+            d.prim_start = hsh[B.sh_begin + k].y;
+            push_records(hmeta, &d, 1);
+        }
+        for (int k = 0; k < B.n_sh; ++k) {
+            const DevShell& d = hs[hsh[B.sh_begin + k].x];
+            push_records(hmeta, hp.data() + d.prim_start, d.n_prims);
+        }
         B.psub = P.psub_elems;
         P.psub_elems += (long long)mp * mp;
         hb[b] = B;
@@ -578,6 +611,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
+         upload(P.block_meta, hmeta) &&
          upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
          upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
@@ -641,7 +675,7 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
         if (!P.class_fits[mode][c]) return kLimit;
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
                    P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
-                   dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
+                   P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
         if (launch_rho(ra, mode, P.classes[c], s) != hipSuccess) return kLaunchFail;
     }
     return kOk;

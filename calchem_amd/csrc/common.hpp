@@ -49,6 +49,10 @@ struct RhoBlock {
     long long psub;  // offset of the block's gathered density (m_pad x m_pad, row-major)
     int pt_begin;  // first sorted point
     int n_pts;
+    // Offset, in 16-byte units, of the block's packed records: RowL[m_pad]
+    // (padding rows with sh = -1), DevShell[n_sh] (prim_start block-local),
+    // double2[n_prim]: what the rho kernel stages in LDS, in LDS order.
+    long long meta;
 };
 
 // One molecule of a plan (a single-molecule plan has one).  Its AOs are the

@@ -13,6 +13,7 @@ struct RhoArgs {
     const RhoBlock* blocks;
     const int2* block_rows; // per block row: (global AO, block-local shell) or (-1, 0)
     const int2* block_sh;   // per block shell: (global shell, block-local primitive offset)
+    const uint4* meta;      // packed per-block records (RhoBlock::meta)
     const double* P64;      // gathered FP64 density blocks (RhoBlock::psub)
     const void* PS;         // gathered density blocks in the precision of the non-leading rows
     double4* rho_out;       // (rho, grad rho) per sorted point, or per caller point when rho_perm

@@ -72,7 +72,8 @@ struct Vec4<float> {
 // ---------------------------------------------------------------------------
 // Dynamic LDS of one rho workgroup, carved in this order (every piece is a
 // multiple of 16 bytes): points, per-wave sums, radial table [shell][point],
-// shells, primitives, rows, FP64 leading-row values, row values.
+// rows, shells, primitives (the block's packed records, copied as one run),
+// FP64 leading-row values, row values.
 __host__ __device__ constexpr size_t rho_lds_fixed() {
     return sizeof(double4) * kRhoPts + sizeof(double) * 16 * kRhoPts;
 }
@@ -127,12 +128,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     q += sizeof(double) * 16 * PTS;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
     q += sizeof(double2) * nsh * PTS;
+    RowL* sRow = reinterpret_cast<RowL*>(q);
+    q += sizeof(RowL) * mp;
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
     q += sizeof(DevShell) * nsh;
     double2* sPrim = reinterpret_cast<double2*>(q);
     q += sizeof(double2) * blk.n_prim;
-    RowL* sRow = reinterpret_cast<RowL*>(q);
-    q += sizeof(RowL) * mp;
     double* sPhi64 = reinterpret_cast<double*>(q);  // [m64][LDP]
     q += HYB ? sizeof(double) * m64 * LDP : 0;
     TS* sPhi = reinterpret_cast<TS*>(q);  // [mp][LDP]
@@ -142,8 +143,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     const int col = lane & 15, kq = lane >> 4;
     using M = Mma<TS>;
     const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
-    stage_meta(a.block_rows + blk.ao_begin, m, mp, a.block_sh + blk.sh_begin, nsh, a.shells, a.prims, a.aos, sRow,
-               sSh, sPrim, tid, kThreads);
+    {
+        // rows, shells and primitives: one coalesced copy of the packed records
+        const uint4* __restrict__ src = a.meta + blk.meta;
+        uint4* dst = reinterpret_cast<uint4*>(sRow);
+        const int n16 = mp + nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + blk.n_prim;
+        for (int i = tid; i < n16; i += kThreads) dst[i] = src[i];
+    }
     const int t0 = tile.y, np = min(PTS, blk.n_pts - t0);
     if (tid < PTS) sPt[tid] = tid < np ? a.pts[blk.pt_begin + t0 + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
@@ -524,12 +530,32 @@ __host__ __device__ constexpr int vgroup_pairs_per_wave() {
     return (vgroup_max_pairs<MAXT>() + VGroupCfg<T, MAXT>::kWaves - 1) / VGroupCfg<T, MAXT>::kWaves;
 }
 
+// Row stride (elements) of the staged phi and Z tiles.  Lane (col, kq) of an
+// MFMA operand reads the 4 consecutive points 4 kq .. 4 kq + 3 of row col with
+// one 16-byte (FP32) or two 16-byte (FP64) LDS reads; with these strides the
+// 16 rows of a tile start on distinct 16-byte bank groups (80 B and 144 B
+// rows), so those reads are free of bank conflicts.
+template <typename T>
+__host__ __device__ constexpr int vgroup_ld() {
+    return sizeof(T) == 4 ? 20 : 18;
+}
+
+// Four consecutive elements from a 16-byte aligned LDS address: one 16-byte
+// read for FP32, two for FP64 (whose 4-vector type would claim 32 bytes of
+// alignment, which the rows do not keep).
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v4f lds_load4(const float* p) { return *reinterpret_cast<const v4f*>(p); }
+__device__ __forceinline__ v4d lds_load4(const double* p) {
+    const v2d lo = reinterpret_cast<const v2d*>(p)[0], hi = reinterpret_cast<const v2d*>(p)[1];
+    return v4d{lo.x, lo.y, hi.x, hi.y};
+}
+
 // LDS before the per-shell tables; the pair table is rounded up to keep
 // 16-byte alignment after it.
 template <typename T, int MAXT>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
     return sizeof(double4) * 2 * kVStage + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
-           sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * (kVPts + 1);
+           sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * vgroup_ld<T>();
 }
 
 template <typename T, int MAXT>
@@ -537,7 +563,9 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NW = VGroupCfg<T, MAXT>::kWaves, NT = NW * 64, PPW = vgroup_pairs_per_wave<T, MAXT>();
     using Rad = typename VGroupCfg<T, MAXT>::Rad;
-    constexpr int LD = kVPts + 1;
+    using V4 = typename Vec4<T>::type;
+    constexpr int LD = vgroup_ld<T>();
+    static_assert((LD * sizeof(T)) % 16 == 0, "phi / Z rows must keep 16-byte alignment");
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh;
     const int T16 = (m + 15) >> 4, mp = T16 << 4;
@@ -636,8 +664,10 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
             sZ[r * LD + p] = q0 + p < G.n_pts ? z : T(0);
         }
         __syncthreads();
+        // MFMA step k takes point 4 kq + k in lane (col, kq): A and B use the
+        // same point order, so the sum over the 16 points is unchanged.
         int cur_i = -1;
-        T zi[4], pi[4];
+        V4 zi, pi;
 #pragma unroll
         for (int w = 0; w < PPW; ++w) {
             const int p = wave * per_wave + w;
@@ -645,18 +675,17 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
             const int2 pr = sPair[p];
             if (w == 0 || pr.x != cur_i) {
                 cur_i = pr.x;
-                const int ri = pr.x + col * LD + kq;
-#pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    zi[k] = sZ[ri + 4 * k];
-                    pi[k] = sP[ri + 4 * k];
-                }
+                const int ri = pr.x + col * LD + 4 * kq;
+                zi = lds_load4(sZ + ri);
+                pi = lds_load4(sP + ri);
             }
-            const int rj = pr.y + col * LD + kq;
+            const int rj = pr.y + col * LD + 4 * kq;
+            const V4 pj = lds_load4(sP + rj);
+            const V4 zj = lds_load4(sZ + rj);
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                acc[w] = M::op(zi[k], sP[rj + 4 * k], acc[w]);
-                acc[w] = M::op(pi[k], sZ[rj + 4 * k], acc[w]);
+                acc[w] = M::op(zi[k], pj[k], acc[w]);
+                acc[w] = M::op(pi[k], zj[k], acc[w]);
             }
         }
     }
