@@ -600,6 +600,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (P.n_group_list > 0 &&
         vgroup_lds_bytes(false, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) > P.lds_limit)
         return kLimit;
+    if (P.max_mpad_rho > 16384) return kLimit;  // the gather stages a block's AO indices in LDS
     P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
     const size_t need = sizeof(double) * (size_t)scratch + sizeof(RedEntry) * hgg.size() +
@@ -667,7 +668,7 @@ void resolve_profile(Plan& P) {
 int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark) {
     GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
-    if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
+    if (launch_gather_p(ga, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
     if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
@@ -1167,7 +1168,7 @@ int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double* Pa, const
         const double* dP = pass == 0 ? dPa : dPb;
         GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP,
                       P.psub64.as<double>(), nullptr, 0};
-        if (launch_gather_p(ga, P.n_blocks, s) != hipSuccess) return kLaunchFail;
+        if (launch_gather_p(ga, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
         for (size_t c = 0; c < P.classes.size(); ++c) {
             const RhoClass& cl = P.classes[c];
             GradAoArgs g{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),

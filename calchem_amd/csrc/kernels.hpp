@@ -188,7 +188,8 @@ hipError_t launch_reduce_en(const double* e, const double* ne, const MolInfo* mo
                             hipStream_t s);
 hipError_t launch_to_float(const double* x, float* y, long long n, hipStream_t s);
 hipError_t launch_fill_zero(double* x, long long n, hipStream_t s);
-hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, hipStream_t s);
+// max_mpad: the largest m_pad of the blocks (at most 16384).
+hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, int max_mpad, hipStream_t s);
 hipError_t launch_significance(const SigArgs& a, hipStream_t s);
 
 // Grid construction (grid.hip).

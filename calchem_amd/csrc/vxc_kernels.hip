@@ -143,6 +143,23 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     const int col = lane & 15, kq = lane >> 4;
     using M = Mma<TS>;
     const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
+    using V4 = typename Vec4<TS>::type;
+    // The first row tile's first density chunks are loaded now, so that their
+    // latency passes during the staging and AO evaluation below (the loads
+    // are the ones contract() makes for that tile; see there for the tail).
+    const int k16 = (m64 + 15) & ~15;
+    V4 pre0{}, pre1{};
+    double pre64[4] = {0.0, 0.0, 0.0, 0.0};
+    if (wave < T16) {
+        const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>((wave << 4) + col) * mp;
+        pre0 = *reinterpret_cast<const V4*>(Arow + k16 + 4 * kq);
+        pre1 = *reinterpret_cast<const V4*>(Arow + k16 + 16 + 4 * kq);
+        if (HYB && m64 >= 16) {
+            const double* __restrict__ A64 = a.P64 + blk.psub + static_cast<long long>((wave << 4) + f64_row_as_f32(col)) * mp;
+#pragma unroll
+            for (int s = 0; s < 4; ++s) pre64[s] = A64[4 * s + kq];
+        }
+    }
     {
         // rows, shells and primitives: one coalesced copy of the packed records
         const uint4* __restrict__ src = a.meta + blk.meta;
@@ -204,7 +221,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         __syncthreads();
 
         // T for row tile it: this lane ends up with T[row(kq, v)][col].
-        auto contract = [&](int it, typename M::acc& acc, v4d& acc64) {
+        auto contract = [&](int it, typename M::acc& acc, v4d& acc64, bool first) {
             const int i0 = it << 4;
             acc = M::zero();
             if constexpr (HYB) {
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
                 for (; k + 16 <= m64; k += 16) {
                     double av[4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) av[s] = Arow[k + 4 * s + kq];
+                    for (int s = 0; s < 4; ++s) av[s] = first && k == 0 ? pre64[s] : Arow[k + 4 * s + kq];
 #pragma unroll
                     for (int s = 0; s < 4; ++s)
                         acc64 = Mma<double>::op(av[s], sPhi64[(k + 4 * s + kq) * LDP + col], acc64);
@@ -235,9 +252,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>(i0 + col) * mp;
             int k = m64;
             for (; k < mp && (k & 15); k += 4) acc = M::op(Arow[k + kq], sPhi[(k + kq) * LDP + col], acc);
-            using V4 = typename Vec4<TS>::type;
-            V4 a0 = opaque(*reinterpret_cast<const V4*>(Arow + k + 4 * kq));
-            V4 a1 = opaque(*reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq));
+            // here k == k16 (m64 rounded up to 16, which never exceeds mp)
+            V4 a0 = first ? pre0 : opaque(*reinterpret_cast<const V4*>(Arow + k + 4 * kq));
+            V4 a1 = first ? pre1 : opaque(*reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq));
             for (; k < mp; k += 16) {
                 const V4 a2 = *reinterpret_cast<const V4*>(Arow + k + 32 + 4 * kq);
                 const int kb = k + 4 * kq;
@@ -267,7 +284,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             if (it < T16) {
                 typename M::acc acc;
                 v4d acc64;
-                contract(it, acc, acc64);
+                contract(it, acc, acc64, j == 0);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, cache[j][v]);
             }
@@ -275,7 +292,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
             typename M::acc acc;
             v4d acc64;
-            contract(it, acc, acc64);
+            contract(it, acc, acc64, false);
 #pragma unroll
             for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, eval(it, v));
         }
@@ -780,19 +797,31 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
     if (i < n) y[i] = static_cast<float>(x[i]);
 }
 
+// One workgroup per block: its m_pad x m_pad density sub-block, row by row
+// (wave w takes rows w, w + 4, ...; lanes run along the row, so the stores
+// are contiguous).  The block's AO indices are staged in LDS first (dynamic
+// LDS of 4 x the largest m_pad of the launch).
 __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
+    extern __shared__ int sIdx[];
     const RhoBlock B = a.blocks[blockIdx.x];
     const MolInfo mol = a.mols[B.mol];
     const double* __restrict__ P = a.P + mol.p_off;
-    const int mp = B.m_pad;
-    for (int idx = threadIdx.x; idx < mp * mp; idx += kThreads) {
-        const int i = idx / mp, j = idx % mp;
-        const int ri = a.block_rows[B.ao_begin + i].x, rj = a.block_rows[B.ao_begin + j].x;
-        const double v = (ri >= 0 && rj >= 0)
-                             ? P[static_cast<long long>(ri - mol.ao_base) * mol.n_ao + (rj - mol.ao_base)]
-                             : 0.0;
-        if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[B.psub + idx] = v;
-        if (a.out32) a.out32[B.psub + idx] = static_cast<float>(v);
+    const int mp = B.m_pad, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int r = threadIdx.x; r < mp; r += kThreads) {
+        const int ao = a.block_rows[B.ao_begin + r].x;
+        sIdx[r] = ao >= 0 ? ao - mol.ao_base : -1;
+    }
+    __syncthreads();
+    for (int i = wave; i < mp; i += kThreads / 64) {
+        const int ri = sIdx[i];
+        const double* __restrict__ Prow = P + static_cast<long long>(max(ri, 0)) * mol.n_ao;
+        const long long o = B.psub + static_cast<long long>(i) * mp;
+        for (int j = lane; j < mp; j += 64) {
+            const int rj = sIdx[j];
+            const double v = (ri >= 0 && rj >= 0) ? Prow[rj] : 0.0;
+            if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[o + j] = v;
+            if (a.out32) a.out32[o + j] = static_cast<float>(v);
+        }
     }
 }
 
@@ -964,9 +993,10 @@ hipError_t launch_fill_zero(double* x, long long n, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, hipStream_t s) {
+hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, int max_mpad, hipStream_t s) {
     if (n_blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather_p, dim3(n_blocks), dim3(kThreads), 0, s, a);
+    if (max_mpad > 16384) return hipErrorInvalidValue;  // 64 KiB of row indices
+    hipLaunchKernelGGL(k_gather_p, dim3(n_blocks), dim3(kThreads), sizeof(int) * (size_t)std::max(max_mpad, 1), s, a);
     return hipGetLastError();
 }
 
