@@ -414,7 +414,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         B.ao_begin = static_cast<int>(hrows.size());
         B.m = m;
         B.m_pad = mp;
-        B.m64 = std::min(round_up(static_cast<int>(big.size()), 4), mp);
+        B.m64 = std::min(round_up(static_cast<int>(big.size()), 16), mp);  // whole 16-column chunks
         B.sh_begin = static_cast<int>(hsh.size());
         std::vector<int> local(n_shells, -1);
         for (int mu : rows) {

@@ -36,7 +36,7 @@ struct DevAo {
 
 // rho-stage block: a spatially compact set of points and the AOs that are
 // significant at any of them.  Rows whose |phi| exceeds the mixed-precision
-// threshold somewhere in the block come first (m64 of them, rounded up to 4).
+// threshold somewhere in the block come first (m64 of them, rounded up to 16).
 struct RhoBlock {
     int ao_begin;  // offset into the block row list
     int m;         // significant AOs

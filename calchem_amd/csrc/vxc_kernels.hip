@@ -94,13 +94,6 @@ __device__ __forceinline__ int xcd_order(int i, int n) {
 // (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
 __device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >> 2); }
 
-// x unchanged, through a register the compiler cannot see into.
-template <typename V>
-__device__ __forceinline__ V opaque(V x) {
-    asm("" : "+v"(x));
-    return x;
-}
-
 // One workgroup (4 waves) per tile of 16 points of a rho block.  The block's
 // rows, shells and primitives are staged in LDS; radial sums once per (shell,
 // point); then wave w evaluates (phi, grad phi) for its 16-row tiles
@@ -147,13 +140,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     // The first row tile's first density chunks are loaded now, so that their
     // latency passes during the staging and AO evaluation below (the loads
     // are the ones contract() makes for that tile; see there for the tail).
-    const int k16 = (m64 + 15) & ~15;
     V4 pre0{}, pre1{};
     double pre64[4] = {0.0, 0.0, 0.0, 0.0};
     if (wave < T16) {
         const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>((wave << 4) + col) * mp;
-        pre0 = *reinterpret_cast<const V4*>(Arow + k16 + 4 * kq);
-        pre1 = *reinterpret_cast<const V4*>(Arow + k16 + 16 + 4 * kq);
+        pre0 = *reinterpret_cast<const V4*>(Arow + m64 + 4 * kq);
+        pre1 = *reinterpret_cast<const V4*>(Arow + m64 + 16 + 4 * kq);
         if (HYB && m64 >= 16) {
             const double* __restrict__ A64 = a.P64 + blk.psub + static_cast<long long>((wave << 4) + f64_row_as_f32(col)) * mp;
 #pragma unroll
@@ -231,7 +223,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
                 const double* __restrict__ Arow =
                     a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
                 int k = 0;
-                for (; k + 16 <= m64; k += 16) {
+                for (; k < m64; k += 16) {
                     double av[4];
 #pragma unroll
                     for (int s = 0; s < 4; ++s) av[s] = first && k == 0 ? pre64[s] : Arow[k + 4 * s + kq];
@@ -239,35 +231,49 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
                     for (int s = 0; s < 4; ++s)
                         acc64 = Mma<double>::op(av[s], sPhi64[(k + 4 * s + kq) * LDP + col], acc64);
                 }
-                for (; k < m64; k += 4) acc64 = Mma<double>::op(Arow[k + kq], sPhi64[(k + kq) * LDP + col], acc64);
             }
             // K in chunks of 16 whose slots are permuted so that lane (i, q)
             // reads four consecutive density entries per load: step s of a
-            // chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].  Density
-            // loads run two chunks ahead (reads past the row end land in the
-            // next row or the buffer tail and are not used); the first two
-            // enter the rotation through an empty asm so that the compiler
-            // cannot fold the rotated registers back into one load at the
-            // point of use.
-            const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>(i0 + col) * mp;
-            int k = m64;
-            for (; k < mp && (k & 15); k += 4) acc = M::op(Arow[k + kq], sPhi[(k + kq) * LDP + col], acc);
-            // here k == k16 (m64 rounded up to 16, which never exceeds mp)
-            V4 a0 = first ? pre0 : opaque(*reinterpret_cast<const V4*>(Arow + k + 4 * kq));
-            V4 a1 = first ? pre1 : opaque(*reinterpret_cast<const V4*>(Arow + k + 16 + 4 * kq));
-            for (; k < mp; k += 16) {
-                const V4 a2 = *reinterpret_cast<const V4*>(Arow + k + 32 + 4 * kq);
-                const int kb = k + 4 * kq;
-                TS b[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) b[s] = sPhi[(kb + s) * LDP + col];
-                acc = M::op(a0[0], b[0], acc);
-                acc = M::op(a0[1], b[1], acc);
-                acc = M::op(a0[2], b[2], acc);
-                acc = M::op(a0[3], b[3], acc);
-                a0 = a1;
-                a1 = a2;
+            // chunk pairs A[i][c + 4 q + s] with Phi[c + 4 q + s].  The
+            // density loads run two chunks ahead through a ring of three
+            // registers, unrolled by three so that no register is copied
+            // (a copy would wait for its load at the copy); reads past the
+            // row end land in the next row or the buffer tail and are not
+            // used.  m64 is a multiple of 16, so the chunks start there.
+            const TS* __restrict__ Ap = PS + blk.psub + static_cast<long long>(i0 + col) * mp + m64 + 4 * kq;
+            const TS* __restrict__ Bp = sPhi + (m64 + 4 * kq) * LDP + col;
+            const int nch = (mp - m64) >> 4;
+            auto chunk = [&](const V4& av, int c) {
+                const TS* b = Bp + c * 16 * LDP;
+                const TS b0 = b[0], b1 = b[LDP], b2 = b[2 * LDP], b3 = b[3 * LDP];
+                acc = M::op(av[0], b0, acc);
+                acc = M::op(av[1], b1, acc);
+                acc = M::op(av[2], b2, acc);
+                acc = M::op(av[3], b3, acc);
+            };
+            auto load = [&](int c) { return *reinterpret_cast<const V4*>(Ap + 16 * c); };
+            V4 r0 = first ? pre0 : load(0);
+            V4 r1 = first ? pre1 : load(1);
+            V4 r2;
+            int c = 0;
+            // The scheduling barriers keep each load ahead of the chunk that
+            // follows it (the scheduler would otherwise sink it to its use).
+            for (; c + 3 <= nch; c += 3) {
+                r2 = load(c + 2);
+                __builtin_amdgcn_sched_barrier(0);
+                chunk(r0, c);
+                __builtin_amdgcn_sched_barrier(0);
+                r0 = load(c + 3);
+                __builtin_amdgcn_sched_barrier(0);
+                chunk(r1, c + 1);
+                __builtin_amdgcn_sched_barrier(0);
+                r1 = load(c + 4);
+                __builtin_amdgcn_sched_barrier(0);
+                chunk(r2, c + 2);
+                __builtin_amdgcn_sched_barrier(0);
             }
+            if (c < nch) chunk(r0, c);
+            if (c + 1 < nch) chunk(r1, c + 1);
         };
         double sr = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
         auto accumulate = [&](const typename M::acc& acc, const v4d& acc64, int v, const AoVal& f) {
