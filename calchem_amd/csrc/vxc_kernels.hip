@@ -216,6 +216,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         auto contract = [&](int it, typename M::acc& acc, v4d& acc64, bool first) {
             const int i0 = it << 4;
             acc = M::zero();
+            // The first two density chunks of the rows in the precision TS are
+            // requested before the FP64 leading part, so that they arrive
+            // while it runs.
+            const TS* __restrict__ Ap = PS + blk.psub + static_cast<long long>(i0 + col) * mp + m64 + 4 * kq;
+            auto load = [&](int c) { return *reinterpret_cast<const V4*>(Ap + 16 * c); };
+            V4 r0 = first ? pre0 : load(0);
+            V4 r1 = first ? pre1 : load(1);
+            __builtin_amdgcn_sched_barrier(0);
             if constexpr (HYB) {
                 acc64 = Mma<double>::zero();
                 // Leading rows (large |phi|) in FP64 with the FP64 density
@@ -240,20 +248,17 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             // (a copy would wait for its load at the copy); reads past the
             // row end land in the next row or the buffer tail and are not
             // used.  m64 is a multiple of 16, so the chunks start there.
-            const TS* __restrict__ Ap = PS + blk.psub + static_cast<long long>(i0 + col) * mp + m64 + 4 * kq;
             const TS* __restrict__ Bp = sPhi + (m64 + 4 * kq) * LDP + col;
             const int nch = (mp - m64) >> 4;
             auto chunk = [&](const V4& av, int c) {
                 const TS* b = Bp + c * 16 * LDP;
                 const TS b0 = b[0], b1 = b[LDP], b2 = b[2 * LDP], b3 = b[3 * LDP];
+                __builtin_amdgcn_sched_barrier(0);  // the four reads in flight together
                 acc = M::op(av[0], b0, acc);
                 acc = M::op(av[1], b1, acc);
                 acc = M::op(av[2], b2, acc);
                 acc = M::op(av[3], b3, acc);
             };
-            auto load = [&](int c) { return *reinterpret_cast<const V4*>(Ap + 16 * c); };
-            V4 r0 = first ? pre0 : load(0);
-            V4 r1 = first ? pre1 : load(1);
             V4 r2;
             int c = 0;
             // The scheduling barriers keep each load ahead of the chunk that
