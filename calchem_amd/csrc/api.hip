@@ -565,6 +565,14 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (size_t i = 0; i < hgrows.size(); ++i) hgao[i] = hgrows[i].x;
     P.n_pairs = static_cast<int>(hpairs.size());
     P.n_group_list = static_cast<int>(hglist.size());
+    // Launch the register-resident groups costliest first (tile pairs x
+    // points), so that the last workgroups to start are short ones.
+    std::stable_sort(hglist.begin(), hglist.end(), [&](int x, int y) {
+        const auto cost = [&](int g) {
+            return (double)hg[g].n_tiles * (hg[g].n_tiles + 1) * (double)round_up(hg[g].n_pts, kVPts);
+        };
+        return cost(x) > cost(y);
+    });
     // CSR: for each AO the groups holding it, ascending, with its row there.
     std::vector<int> hgb(n_ao + 1, 0);
     std::vector<RedEntry> hgg;
