@@ -459,19 +459,23 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         P.flops_rho_fp64_rows += 2.0 * m * (double)big.size() * nb;
     }
     // Tiles of 16 points grouped into launch classes by the block's m_pad so
-    // each launch sizes its LDS for similar blocks.
+    // each launch sizes its LDS for similar blocks.  Within a class the blocks
+    // are dealt to kXcds segments of about equal cost (largest first, each to
+    // the segment with the least cost so far), each segment ordered costliest
+    // block first; a block's tiles stay consecutive.  Cost of a tile: m_pad^2
+    // plus a fixed part for its staging and AO work (that of m_pad = 128).
     std::vector<int2> htiles;
     {
         const int ncls = std::max((P.max_mpad_rho + 63) / 64, 1);
-        std::vector<std::vector<int2>> bycls(ncls);
-        std::vector<RhoClass> cls(ncls, RhoClass{0, 0, 0, 0, 0, 0});
+        std::vector<std::vector<int>> bycls(ncls);
+        std::vector<RhoClass> cls(ncls, RhoClass{});
         for (int b = 0; b < P.n_blocks; ++b) {
             const int nb = leaves[b].second - leaves[b].first;
             hb[b].pt_begin = leaves[b].first;
             hb[b].n_pts = nb;
             if (hb[b].m == 0) continue;
             const int c = (hb[b].m_pad - 1) / 64;
-            for (int o = 0; o < nb; o += kRhoPts) bycls[c].push_back(make_int2(b, o));
+            bycls[c].push_back(b);
             P.n_rho_tiles += (nb + kRhoPts - 1) / kRhoPts;
             P.flops_rho_pad += 2.0 * hb[b].m_pad * (double)hb[b].m_pad * round_up(nb, kRhoPts);
             cls[c].max_mpad = std::max(cls[c].max_mpad, hb[b].m_pad);
@@ -479,11 +483,28 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             cls[c].max_m64 = std::max(cls[c].max_m64, hb[b].m64);
             cls[c].max_prim = std::max(cls[c].max_prim, hb[b].n_prim);
         }
+        auto block_cost = [&](int b) {
+            return (double)((hb[b].n_pts + kRhoPts - 1) / kRhoPts) * ((double)hb[b].m_pad * hb[b].m_pad + 16384.0);
+        };
         for (int c = 0; c < ncls; ++c) {
             if (bycls[c].empty()) continue;
+            std::vector<int>& bl = bycls[c];
+            std::stable_sort(bl.begin(), bl.end(), [&](int x, int y) { return block_cost(x) > block_cost(y); });
+            std::vector<std::vector<int>> seg(kXcds);
+            double load[kXcds] = {};
+            for (int b : bl) {
+                const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
+                seg[x].push_back(b);
+                load[x] += block_cost(b);
+            }
             cls[c].tile_begin = static_cast<int>(htiles.size());
-            cls[c].n_tiles = static_cast<int>(bycls[c].size());
-            htiles.insert(htiles.end(), bycls[c].begin(), bycls[c].end());
+            for (int x = 0; x < kXcds; ++x) {
+                cls[c].seg[x] = static_cast<int>(htiles.size()) - cls[c].tile_begin;
+                for (int b : seg[x])
+                    for (int o = 0; o < hb[b].n_pts; o += kRhoPts) htiles.push_back(make_int2(b, o));
+            }
+            cls[c].seg[kXcds] = static_cast<int>(htiles.size()) - cls[c].tile_begin;
+            cls[c].n_tiles = cls[c].seg[kXcds];
             P.classes.push_back(cls[c]);
         }
     }

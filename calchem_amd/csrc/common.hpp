@@ -102,6 +102,7 @@ constexpr int kGroupMaxTiles = 17;   // default 16-row tiles per register-reside
 // Elements after the last gathered density block: the rho kernel's look-ahead
 // loads may read up to 3 chunks of 16 past a row's end (values unused).
 constexpr int kPsubTail = 64;
+constexpr int kXcds = 8;  // accelerator complex dies of an MI355X, each with its own L2
 
 inline bool hip_ok(hipError_t e, const char* what) {
     if (e != hipSuccess) {

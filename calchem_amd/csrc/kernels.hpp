@@ -19,6 +19,7 @@ struct RhoArgs {
     double4* rho_out;       // (rho, grad rho) per sorted point, or per caller point when rho_perm
     const int* perm;        // sorted point -> caller point
     int rho_perm;
+    int seg[kXcds + 1];     // this launch's XCD segments (RhoClass::seg)
 };
 
 // Functional-stage workgroups never straddle molecules: workgroup w covers
@@ -103,9 +104,13 @@ hipError_t launch_grad_sum(const double* part, long long rows, int width, long l
 void becke_pair_tables(int n_atoms, const int* z, const double* xyz, double* inv_r, double* asym);
 
 // Launch class of rho tiles: blocks of similar size share one LDS budget.
+// A launch class of rho tiles.  Its tiles are split into kXcds segments of
+// about equal cost, segment x = [seg[x], seg[x + 1]) (class-local), each a
+// run of whole blocks, costliest first; XCD x walks segment x.
 struct RhoClass {
     int tile_begin, n_tiles;  // range in the class-ordered tile list
     int max_mpad, max_nsh, max_m64, max_prim;
+    int seg[kXcds + 1];
 };
 
 struct VArgs {

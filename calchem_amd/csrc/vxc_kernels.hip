@@ -78,17 +78,11 @@ __host__ __device__ constexpr size_t rho_lds_fixed() {
     return sizeof(double4) * kRhoPts + sizeof(double) * 16 * kRhoPts;
 }
 
-// Workgroups are dispatched round robin over the 8 XCDs (workgroup i to XCD
-// i mod 8), each with its own L2.  This maps workgroup i to work item
-// xcd_order(i, n) so that XCD x walks one contiguous range of the work list:
-// consecutive tiles of a block (which read the same density block) then share
-// an L2.  A bijection of [0, n).
-constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_order(int i, int n) {
-    const int x = i % kXcds, k = i / kXcds;
-    const int q = n / kXcds, r = n % kXcds;
-    return x * q + min(x, r) + k;
-}
+// Workgroups are dispatched round robin over the XCDs (workgroup i to XCD
+// i mod kXcds), each with its own L2.  Workgroup i = kXcds k + x takes tile k
+// of segment x, so XCD x walks its segment in order: consecutive tiles of a
+// block (which read the same density block) share an L2.  Workgroups past
+// the end of their segment exit at once.
 
 // Position in an f64 16x16x4 A tile that makes its result element v of lane
 // (q, j) land on row 4 q + v, the row the f32 instruction gives that slot.
@@ -110,7 +104,9 @@ template <typename TS, bool HYB>
 __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = HYB ? 2 : 3;
-    const int2 tile = a.tiles[xcd_order(blockIdx.x, gridDim.x)];  // (block, first point within it)
+    const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
+    if (t >= a.seg[xcd + 1]) return;
+    const int2 tile = a.tiles[t];  // (block, first point within it)
     const RhoBlock blk = a.blocks[tile.x];
     const int m = blk.m, mp = blk.m_pad, nsh = blk.n_sh, T16 = blk.m_pad >> 4;
     const int m64 = HYB ? blk.m64 : 0;
@@ -881,7 +877,13 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
     if (c.n_tiles <= 0) return hipSuccess;
     RhoArgs b = a;
     b.tiles = a.tiles + c.tile_begin;
-    hipLaunchKernelGGL((k_rho<TS, HYB>), dim3(c.n_tiles), dim3(kThreads), lds, s, b);
+    int longest = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        b.seg[x] = c.seg[x];
+        longest = std::max(longest, c.seg[x + 1] - c.seg[x]);
+    }
+    b.seg[kXcds] = c.seg[kXcds];
+    hipLaunchKernelGGL((k_rho<TS, HYB>), dim3(kXcds * longest), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
