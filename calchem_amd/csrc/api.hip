@@ -73,6 +73,7 @@ struct Plan {
     // Rho launch classes and, per mode (FP64 / mixed), whether each fits LDS.
     std::vector<RhoClass> classes;
     std::vector<char> class_fits[2];
+    bool v_fits_fp32 = true;  // the mixed build's V kernel fits the LDS limit
     double flops_rho = 0, flops_v = 0, flops_rho_pad = 0, flops_v_pad = 0, flops_rho_fp64_rows = 0;
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
@@ -629,6 +630,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (P.n_group_list > 0 &&
         vgroup_lds_bytes(false, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) > P.lds_limit)
         return kLimit;
+    P.v_fits_fp32 = P.n_group_list == 0 ||
+                    vgroup_lds_bytes(true, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) <= P.lds_limit;
     if (P.max_mpad_rho > 16384) return kLimit;  // the gather stages a block's AO indices in LDS
     P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
@@ -715,6 +718,7 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
 // the rest, then the deterministic reduction into the full symmetric V.
 // With fused set this is part of a fused build and records its profiling mark.
 int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s, bool fused) {
+    if (f32 && !P.v_fits_fp32) return kLimit;
     VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
              P.group_sh.as<int2>(), wv, P.perm.as<int>(), wv_perm, P.scratch.as<double>()};

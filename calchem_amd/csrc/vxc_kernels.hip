@@ -524,8 +524,7 @@ __global__ __launch_bounds__(kThreads) void k_vxc(VArgs a) {
 // time, radial sums (per shell) then phi and Z = w v_rho/2 phi + wu . dphi
 // (per row) are staged in LDS, each evaluated once, and every pair takes
 // acc += Z_I Phi_J^T + Phi_I Z_J^T.  FP64: 8 waves, AO evaluation in FP64.
-// FP32 (mixed build): 16 waves (half the pairs per wave, four waves per SIMD)
-// and AO evaluation in FP32.
+// (The mixed build uses k_vxc_group_pipe below: FP32 tiles and AO values.)
 template <typename T, int MAXT>
 struct VGroupCfg;
 template <>
@@ -537,11 +536,6 @@ template <>
 struct VGroupCfg<double, 12> {
     static constexpr int kWaves = 16;  // 5 FP64 pairs per wave: four waves per SIMD
     using Rad = double2;
-};
-template <int MAXT>
-struct VGroupCfg<float, MAXT> {
-    static constexpr int kWaves = 16;
-    using Rad = float2;
 };
 
 template <int MAXT>
@@ -659,11 +653,7 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
             const int sh = idx / kVPts, p = idx % kVPts;
             const double4 x = sPt[p];
             const bool live = q0 + p < G.n_pts;
-            if constexpr (sizeof(T) == 8) {
-                sRad[idx] = live ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
-            } else {
-                sRad[idx] = live ? shell_radial_f(sSh[sh], sPrim, x.x, x.y, x.z) : make_float2(0.0f, 0.0f);
-            }
+            sRad[idx] = live ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
         }
         __syncthreads();
         for (int idx = tid; idx < mp * kVPts; idx += NT) {
@@ -673,16 +663,9 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
             if (row.sh >= 0) {
                 const double4 x = sPt[p];
                 const double4 w = sWv[p];
-                if constexpr (sizeof(T) == 8) {
-                    const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
-                    v = f.v;
-                    z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
-                } else {
-                    const AoValF f = ao_from_radial_row_f(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
-                    v = f.v;
-                    z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx +
-                        static_cast<float>(w.z) * f.gy + static_cast<float>(w.w) * f.gz;
-                }
+                const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
+                v = f.v;
+                z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
             }
             sP[r * LD + p] = v;
             sZ[r * LD + p] = q0 + p < G.n_pts ? z : T(0);
@@ -712,6 +695,164 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
                 acc[w] = M::op(pi[k], zj[k], acc[w]);
             }
         }
+    }
+#pragma unroll
+    for (int w = 0; w < PPW; ++w) {
+        const int p = wave * per_wave + w;
+        if (w >= per_wave || p >= npairs) continue;
+        double* out = a.scratch + G.scratch_off + static_cast<long long>(p) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v) out[M::row(kq, v) * 16 + col] = static_cast<double>(acc[w][v]);
+    }
+}
+
+// Pipelined form of k_vxc_group for the mixed build (FP32 tiles, 16 waves):
+// one barrier per step of 16 points.  Between two barriers the workgroup
+// does four independent things: the MFMA updates of step s (phi / Z buffer
+// s % 2), the AO values of step s + 1 into the other buffer, the radial sums
+// of step s + 2 (radial buffer s % 2) and the loads of the points and wv of
+// step s + 3 (slot (s + 3) % 4 of four).  Each buffer is written in one
+// interval and read in a later one, so the barrier between them is the only
+// ordering needed; waves that finish their MFMA share early go on with AO
+// evaluation, so MFMA and VALU work overlap on the SIMDs.
+constexpr int kVSlots = 4;
+
+template <int MAXT>
+__host__ __device__ constexpr size_t vgroup_pipe_lds_fixed() {
+    return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+           sizeof(RowL) * MAXT * 16 + sizeof(float) * 4 * MAXT * 16 * vgroup_ld<float>();
+}
+
+template <int MAXT>
+__global__ __launch_bounds__(1024) void k_vxc_group_pipe(VArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    using T = float;
+    constexpr int NW = 16, NT = NW * 64, PPW = (vgroup_max_pairs<MAXT>() + NW - 1) / NW;
+    constexpr int LD = vgroup_ld<T>();
+    const VGroup G = a.groups[a.group_list[blockIdx.x]];
+    const int m = G.m, nsh = G.n_sh, npts = G.n_pts;
+    const int T16 = (m + 15) >> 4, mp = T16 << 4;
+    const int npairs = T16 * (T16 + 1) / 2;
+    const int nsteps = (npts + kVPts - 1) / kVPts;
+    unsigned char* q = smem;
+    double4* sPt = reinterpret_cast<double4*>(q);  // [kVSlots][kVPts] points, then [kVSlots][kVPts] wv
+    double4* sWv = sPt + kVSlots * kVPts;
+    q += sizeof(double4) * 2 * kVSlots * kVPts;
+    int2* sPair = reinterpret_cast<int2*>(q);
+    q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
+    RowL* sRow = reinterpret_cast<RowL*>(q);
+    q += sizeof(RowL) * MAXT * 16;
+    T* sPZ = reinterpret_cast<T*>(q);  // buffer b: phi at sPZ + 2 b S, Z at sPZ + (2 b + 1) S
+    constexpr int S = MAXT * 16 * LD;
+    q += sizeof(T) * 4 * S;
+    float2* sRad = reinterpret_cast<float2*>(q);  // buffer b at sRad + b nsh kVPts
+    q += sizeof(float2) * 2 * nsh * kVPts;
+    DevShell* sSh = reinterpret_cast<DevShell*>(q);
+    q += sizeof(DevShell) * nsh;
+    double2* sPrim = reinterpret_cast<double2*>(q);
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+    stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
+               sPrim, tid, NT);
+    const int per_wave = (npairs + NW - 1) / NW;
+    for (int p = tid; p < npairs; p += NT) {
+        int ti = 0;
+        while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
+        const int tj = p - ti * (ti + 1) / 2;
+        sPair[p] = make_int2(ti * 16 * LD, tj * 16 * LD);
+    }
+
+    // points and wv of step st into its slot (zero past the group's points)
+    auto load_points = [&](int st) {
+        if (tid < 2 * kVPts) {
+            const int j = tid % kVPts, g = st * kVPts + j, slot = st % kVSlots;
+            double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (g < npts) {
+                const int sg = G.pt_begin + g;
+                v = tid < kVPts ? a.pts[sg] : a.wv[a.wv_perm ? a.perm[sg] : sg];
+            }
+            (tid < kVPts ? sPt : sWv)[slot * kVPts + j] = v;
+        }
+    };
+    auto radial = [&](int st) {
+        const double4* pt = sPt + (st % kVSlots) * kVPts;
+        float2* rad = sRad + (st & 1) * nsh * kVPts;
+        for (int idx = tid; idx < nsh * kVPts; idx += NT) {
+            const int sh = idx / kVPts, p = idx % kVPts;
+            const double4 x = pt[p];
+            rad[idx] = st * kVPts + p < npts ? shell_radial_f(sSh[sh], sPrim, x.x, x.y, x.z) : make_float2(0.0f, 0.0f);
+        }
+    };
+    auto evaluate = [&](int st) {
+        const double4* pt = sPt + (st % kVSlots) * kVPts;
+        const double4* wv = sWv + (st % kVSlots) * kVPts;
+        const float2* rad = sRad + (st & 1) * nsh * kVPts;
+        T* P = sPZ + 2 * (st & 1) * S;
+        T* Z = P + S;
+        for (int idx = tid; idx < mp * kVPts; idx += NT) {
+            const int r = idx / kVPts, p = idx % kVPts;
+            const RowL row = sRow[r];
+            T v = 0, z = 0;
+            if (row.sh >= 0) {
+                const double4 x = pt[p];
+                const double4 w = wv[p];
+                const AoValF f = ao_from_radial_row_f(sSh[row.sh], row, x.x, x.y, x.z, rad[row.sh * kVPts + p]);
+                v = f.v;
+                z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx + static_cast<float>(w.z) * f.gy +
+                    static_cast<float>(w.w) * f.gz;
+            }
+            P[r * LD + p] = v;
+            Z[r * LD + p] = st * kVPts + p < npts ? z : T(0);
+        }
+    };
+    using M = Mma<T>;
+    using V4 = typename Vec4<T>::type;
+    typename M::acc acc[PPW];
+#pragma unroll
+    for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
+    auto update = [&](int st) {
+        const T* P = sPZ + 2 * (st & 1) * S;
+        const T* Z = P + S;
+        int cur_i = -1;
+        V4 zi, pi;
+#pragma unroll
+        for (int w = 0; w < PPW; ++w) {
+            const int p = wave * per_wave + w;
+            if (w >= per_wave || p >= npairs) continue;
+            const int2 pr = sPair[p];
+            if (w == 0 || pr.x != cur_i) {
+                cur_i = pr.x;
+                const int ri = pr.x + col * LD + 4 * kq;
+                zi = lds_load4(Z + ri);
+                pi = lds_load4(P + ri);
+            }
+            const int rj = pr.y + col * LD + 4 * kq;
+            const V4 pj = lds_load4(P + rj);
+            const V4 zj = lds_load4(Z + rj);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                acc[w] = M::op(zi[k], pj[k], acc[w]);
+                acc[w] = M::op(pi[k], zj[k], acc[w]);
+            }
+        }
+    };
+
+    // prologue: points of steps 0-2, radial sums of steps 0-1, AO values of step 0
+    for (int st = 0; st < 3 && st < nsteps; ++st) load_points(st);
+    __syncthreads();
+    radial(0);
+    if (nsteps > 1) radial(1);
+    __syncthreads();
+    evaluate(0);
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+        update(st);
+        if (st + 1 < nsteps) evaluate(st + 1);
+        if (st + 2 < nsteps) radial(st + 2);
+        if (st + 3 < nsteps) load_points(st + 3);
+        __syncthreads();
     }
 #pragma unroll
     for (int w = 0; w < PPW; ++w) {
@@ -922,9 +1063,14 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
 
 template <typename T, int MAXT>
 static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim) {
-    const size_t rad = sizeof(typename VGroupCfg<T, MAXT>::Rad) * (size_t)((max_nsh * kVPts + 1) & ~1);
-    return vgroup_lds_fixed<T, MAXT>() + rad + sizeof(DevShell) * (size_t)max_nsh +
-           sizeof(double2) * (size_t)max_prim;
+    if constexpr (sizeof(T) == 4) {  // k_vxc_group_pipe
+        return vgroup_pipe_lds_fixed<MAXT>() + sizeof(float2) * 2 * (size_t)max_nsh * kVPts +
+               sizeof(DevShell) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
+    } else {
+        const size_t rad = sizeof(typename VGroupCfg<T, MAXT>::Rad) * (size_t)((max_nsh * kVPts + 1) & ~1);
+        return vgroup_lds_fixed<T, MAXT>() + rad + sizeof(DevShell) * (size_t)max_nsh +
+               sizeof(double2) * (size_t)max_prim;
+    }
 }
 
 size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim) {
@@ -940,16 +1086,23 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
         int dev = 0, lim = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vxc_group<T, MAXT>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, lim);
+        const void* fn;
+        if constexpr (sizeof(T) == 4)
+            fn = reinterpret_cast<const void*>(&k_vxc_group_pipe<MAXT>);
+        else
+            fn = reinterpret_cast<const void*>(&k_vxc_group<T, MAXT>);
+        if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
         attr_set = 1;
     }
     if (n_groups <= 0) return hipSuccess;
-    constexpr int threads = VGroupCfg<T, MAXT>::kWaves * 64;
     const size_t lds = vgroup_lds_bytes_t<T, MAXT>(max_nsh, max_prim);
-    hipLaunchKernelGGL((k_vxc_group<T, MAXT>), dim3(n_groups), dim3(threads), lds, s, a);
+    if constexpr (sizeof(T) == 4) {
+        hipLaunchKernelGGL((k_vxc_group_pipe<MAXT>), dim3(n_groups), dim3(1024), lds, s, a);
+    } else {
+        constexpr int threads = VGroupCfg<T, MAXT>::kWaves * 64;
+        hipLaunchKernelGGL((k_vxc_group<T, MAXT>), dim3(n_groups), dim3(threads), lds, s, a);
+    }
     return hipGetLastError();
 }
 
