@@ -866,9 +866,10 @@ __global__ __launch_bounds__(1024) void k_vxc_group_pipe(VArgs a) {
 
 // One workgroup per AO row mu: V[mu][nu] (nu <= mu) = sum over the groups
 // holding mu of row li(mu) of that group's lower tiles, scattered through the
-// group's row list.  Wave w of the workgroup takes entries w, w + W, ... in
-// ascending group order into its own LDS copy of the row (a wave's LDS
-// operations execute in order, so its lanes need no barrier between groups);
+// group's row list.  Wave w of the workgroup sums the w-th of W contiguous
+// runs of the entries (ascending group order) into its own LDS copy of the
+// row (a wave's LDS operations execute in order, so its lanes need no barrier
+// between groups);
 // the W copies are then added in wave order, the row written to V and
 // mirrored (V is exactly symmetric).  Group rows are in ascending AO order, so
 // nu <= mu is lj <= li.  Deterministic for a given plan.
@@ -882,14 +883,49 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
     for (int j = tid; j < nw * len; j += blockDim.x) srow[j] = 0.0;
     __syncthreads();
     double* __restrict__ my = srow + w * len;
-    const int e1 = a.ao_gbegin[mu + 1];
-    for (int e = a.ao_gbegin[mu] + w; e < e1; e += nw) {
-        const RedEntry r = a.entries[e];
-        const int li = r.li_ts & 0xffff, ts = r.li_ts >> 16, mask = (1 << ts) - 1;
-        const double* __restrict__ tile = a.scratch + r.base;
-        const int* __restrict__ rows = a.group_ao + r.rows;
-        for (int lj = lane; lj <= li; lj += 64)
-            my[rows[lj] - base] += tile[(static_cast<long long>(lj >> ts) << (2 * ts)) + (lj & mask)];
+    // Wave w takes the w-th of nw contiguous runs of mu's entries, four at a
+    // time: the row indices and tile values of the four are requested
+    // together (one memory round trip), then added in entry order.
+    const int g0 = a.ao_gbegin[mu], e1 = a.ao_gbegin[mu + 1];
+    const int per = (e1 - g0 + nw - 1) / nw;
+    const int eb = g0 + w * per, ee = min(eb + per, e1);
+    constexpr int B = 4;
+    for (int e0 = eb; e0 < ee; e0 += B) {
+        int li[B], ts[B], rows_at[B];
+        long long tbase[B];
+        int maxli = -1;
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            li[u] = -1;
+            ts[u] = 0;
+            rows_at[u] = 0;
+            tbase[u] = 0;
+            if (e0 + u < ee) {
+                const RedEntry r = a.entries[e0 + u];
+                li[u] = r.li_ts & 0xffff;
+                ts[u] = r.li_ts >> 16;
+                rows_at[u] = r.rows;
+                tbase[u] = r.base;
+                maxli = max(maxli, li[u]);
+            }
+        }
+        for (int lj = lane; lj <= maxli; lj += 64) {
+            int c[B];
+            double v[B];
+#pragma unroll
+            for (int u = 0; u < B; ++u) {
+                c[u] = -1;
+                v[u] = 0.0;
+                if (lj <= li[u]) {
+                    c[u] = a.group_ao[rows_at[u] + lj];
+                    v[u] = a.scratch[tbase[u] + (static_cast<long long>(lj >> ts[u]) << (2 * ts[u])) +
+                                     (lj & ((1 << ts[u]) - 1))];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < B; ++u)
+                if (c[u] >= 0) my[c[u] - base] += v[u];
+        }
     }
     __syncthreads();
     const long long n = mol.n_ao;
