@@ -883,9 +883,9 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
     for (int j = tid; j < nw * len; j += blockDim.x) srow[j] = 0.0;
     __syncthreads();
     double* __restrict__ my = srow + w * len;
-    // Wave w takes the w-th of nw contiguous runs of mu's entries, four at a
-    // time: the row indices and tile values of the four are requested
-    // together (one memory round trip), then added in entry order.
+    // Wave w takes the w-th of nw contiguous runs of mu's entries, B at a
+    // time: the row indices and tile values of the B are requested together
+    // (one memory round trip), then added in entry order.
     const int g0 = a.ao_gbegin[mu], e1 = a.ao_gbegin[mu + 1];
     const int per = (e1 - g0 + nw - 1) / nw;
     const int eb = g0 + w * per, ee = min(eb + per, e1);
@@ -981,10 +981,12 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
     if (i < n) y[i] = static_cast<float>(x[i]);
 }
 
-// One workgroup per block: its m_pad x m_pad density sub-block, row by row
-// (wave w takes rows w, w + 4, ...; lanes run along the row, so the stores
-// are contiguous).  The block's AO indices are staged in LDS first (dynamic
-// LDS of 4 x the largest m_pad of the launch).
+// One workgroup per block: its m_pad x m_pad density sub-block, four rows
+// per wave at a time (wave w takes rows 4 w .. 4 w + 3, then 16 further, and
+// so on; lanes run along the rows, so the stores are contiguous, and the
+// four rows' loads of a column chunk are in flight together).  The block's
+// AO indices are staged in LDS first (dynamic LDS of 4 x the largest m_pad
+// of the launch).
 __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
     extern __shared__ int sIdx[];
     const RhoBlock B = a.blocks[blockIdx.x];
@@ -996,15 +998,23 @@ __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
         sIdx[r] = ao >= 0 ? ao - mol.ao_base : -1;
     }
     __syncthreads();
-    for (int i = wave; i < mp; i += kThreads / 64) {
-        const int ri = sIdx[i];
-        const double* __restrict__ Prow = P + static_cast<long long>(max(ri, 0)) * mol.n_ao;
-        const long long o = B.psub + static_cast<long long>(i) * mp;
+    constexpr int R = 4;
+    for (int i0 = wave * R; i0 < mp; i0 += R * (kThreads / 64)) {  // mp is a multiple of 16
+        int ri[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) ri[u] = sIdx[i0 + u];
         for (int j = lane; j < mp; j += 64) {
             const int rj = sIdx[j];
-            const double v = (ri >= 0 && rj >= 0) ? Prow[rj] : 0.0;
-            if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[o + j] = v;
-            if (a.out32) a.out32[o + j] = static_cast<float>(v);
+            double v[R];
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+                v[u] = (ri[u] >= 0 && rj >= 0) ? P[static_cast<long long>(ri[u]) * mol.n_ao + rj] : 0.0;
+#pragma unroll
+            for (int u = 0; u < R; ++u) {
+                const long long o = B.psub + static_cast<long long>(i0 + u) * mp + j;
+                if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[o] = v[u];
+                if (a.out32) a.out32[o] = static_cast<float>(v[u]);
+            }
         }
     }
 }
