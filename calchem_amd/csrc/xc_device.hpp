@@ -150,13 +150,12 @@ __device__ __forceinline__ XcOut b88_correction(double ra, double rb, double gaa
 }
 
 // ---- LYP (Miehlich 1989 gradient-only form) ----
-// Bracket S(ra, rb) = S_TF + A_aa g_aa + A_ab g_ab + A_bb g_bb and dS/d(ra).
-__device__ __forceinline__ void lyp_bracket(double ra, double rb, double dl, double ddl, double gaa,
-                                            double gab, double gbb, double& S, double& dS, double& Aaa,
-                                            double& Aab, double& Abb) {
+// Bracket S(ra, rb) = S_TF + A_aa g_aa + A_ab g_ab + A_bb g_bb and dS/d(ra);
+// ra83, rb83 = ra^(8/3), rb^(8/3).
+__device__ __forceinline__ void lyp_bracket(double ra, double rb, double ra83, double rb83, double dl, double ddl,
+                                            double gaa, double gab, double gbb, double& S, double& dS,
+                                            double& Aaa, double& Aab, double& Abb) {
     const double rho = ra + rb, rho2 = rho * rho;
-    const double ra83 = ra > 0.0 ? pow(ra, 8.0 / 3.0) : 0.0;
-    const double rb83 = rb > 0.0 ? pow(rb, 8.0 / 3.0) : 0.0;
     const double stf = kLypK0 * ra * rb * (ra83 + rb83);
     const double dstf = kLypK0 * ((11.0 / 3.0) * ra83 * rb + rb * rb83);
     const double k47 = (47.0 - 7.0 * dl) / 18.0;
@@ -185,7 +184,8 @@ __device__ __forceinline__ XcOut lyp_correlation(double ra, double rb, double ga
     const double rho = ra + rb;
     if (rho < kRhoCut) return o;
     constexpr double A = 0.04918, B = 0.132, C = 0.2533, D = 0.349;
-    const double rm13 = pow(rho, -1.0 / 3.0);
+    // powers of one third through cbrt (cheaper than pow)
+    const double c = cbrt(rho), rm13 = 1.0 / c;
     const double den = 1.0 + D * rm13;
     const double g0 = -4.0 * A / (rho * den);
     const double dg0 = -g0 * (1.0 + (2.0 / 3.0) * D * rm13) / (rho * den);
@@ -193,13 +193,15 @@ __device__ __forceinline__ XcOut lyp_correlation(double ra, double rb, double ga
     o.va = dg0 * ra * rb + g0 * rb;
     o.vb = dg0 * ra * rb + g0 * ra;
     if (C * rm13 > 500.0) return o;  // omega underflows: only the local term survives
-    const double om = exp(-C * rm13) / den * pow(rho, -11.0 / 3.0);
+    const double om = exp(-C * rm13) / den / (rho * rho * rho * c * c);  // rho^(-11/3)
     const double dl = C * rm13 + D * rm13 / den;
     const double dom = om * (dl - 11.0) / (3.0 * rho);
     const double ddl = -(1.0 / 3.0) * (rm13 / rho) * (C + D / (den * den));
     double S, dSa, Aaa, Aab, Abb, S2, dSb, t0, t1, t2;
-    lyp_bracket(ra, rb, dl, ddl, gaa, gab, gbb, S, dSa, Aaa, Aab, Abb);
-    lyp_bracket(rb, ra, dl, ddl, gbb, gab, gaa, S2, dSb, t0, t1, t2);
+    const double ca = ra > 0.0 ? cbrt(ra) : 0.0, cb = rb > 0.0 ? cbrt(rb) : 0.0;
+    const double ra83 = ra * ra * ca * ca, rb83 = rb * rb * cb * cb;
+    lyp_bracket(ra, rb, ra83, rb83, dl, ddl, gaa, gab, gbb, S, dSa, Aaa, Aab, Abb);
+    lyp_bracket(rb, ra, rb83, ra83, dl, ddl, gbb, gab, gaa, S2, dSb, t0, t1, t2);
     const double ab = A * B;
     o.f -= ab * om * S;
     o.va -= ab * (dom * S + om * dSa);
