@@ -105,16 +105,22 @@ __device__ __forceinline__ XcOut vwn_correlation(double ra, double rb, const Vwn
     const double rs = cbrt(kRsK / rho);
     const double x = sqrt(rs);
     const double zeta = clamp_pm1((ra - rb) / rho);
-    double ep, dp, ef, df, ea, da, fz, dfz, ifdd;
+    double ep, dp;
     vwn_fit(x, P.para, ep, dp);
-    vwn_fit(x, P.ferro, ef, df);
-    vwn_fit(x, P.alpha, ea, da);
-    spin_f(zeta, fz, dfz, ifdd);
-    const double z3 = zeta * zeta * zeta, z4 = z3 * zeta, w4 = 1.0 - z4;
-    const double dfe = ef - ep, dfd = df - dp;
-    const double ec = ep + ea * fz * ifdd * w4 + dfe * fz * z4;
-    const double rsdec = dp + da * fz * ifdd * w4 + dfd * fz * z4;
-    const double dz = ea * ifdd * (dfz * w4 - 4.0 * z3 * fz) + dfe * (dfz * z4 + 4.0 * z3 * fz);
+    // zeta = 0 (every point of a restricted density): f(0) = f'(0) = 0, so the
+    // ferromagnetic and spin-stiffness channels add exactly nothing
+    double ec = ep, rsdec = dp, dz = 0.0;
+    if (zeta != 0.0) {
+        double ef, df, ea, da, fz, dfz, ifdd;
+        vwn_fit(x, P.ferro, ef, df);
+        vwn_fit(x, P.alpha, ea, da);
+        spin_f(zeta, fz, dfz, ifdd);
+        const double z3 = zeta * zeta * zeta, z4 = z3 * zeta, w4 = 1.0 - z4;
+        const double dfe = ef - ep, dfd = df - dp;
+        ec = ep + ea * fz * ifdd * w4 + dfe * fz * z4;
+        rsdec = dp + da * fz * ifdd * w4 + dfd * fz * z4;
+        dz = ea * ifdd * (dfz * w4 - 4.0 * z3 * fz) + dfe * (dfz * z4 + 4.0 * z3 * fz);
+    }
     const double base = ec - rsdec / 3.0;
     o.f = rho * ec;
     o.va = base + (1.0 - zeta) * dz;
@@ -267,17 +273,23 @@ __device__ __forceinline__ XcOut pbe_correlation(double ra, double rb, double ga
     const double rs = cbrt(kRsK / rho);
     const double drs = -rs / (3.0 * rho);
     const double zeta = clamp_pm1((ra - rb) / rho);
-    double e0, de0, e1, de1, ma, dma, fz, dfz, ifdd;
+    double e0, de0;
     pw92(rs, kPwUnpol, e0, de0);
-    pw92(rs, kPwPol, e1, de1);
-    pw92(rs, kPwAlpha, ma, dma);
-    spin_f(zeta, fz, dfz, ifdd);
-    const double ac = -ma, dac = -dma;
-    const double z3 = zeta * zeta * zeta, z4 = z3 * zeta, w4 = 1.0 - z4;
-    const double ec = e0 + ac * fz * ifdd * w4 + (e1 - e0) * fz * z4;
-    const double dec_rs = de0 + dac * fz * ifdd * w4 + (de1 - de0) * fz * z4;
+    // zeta = 0: f(0) = f'(0) = 0, the polarized and spin-stiffness fits add
+    // exactly nothing (and phi = 1 below)
+    double ec = e0, dec_rs = de0, dec_z = 0.0;
+    if (zeta != 0.0) {
+        double e1, de1, ma, dma, fz, dfz, ifdd;
+        pw92(rs, kPwPol, e1, de1);
+        pw92(rs, kPwAlpha, ma, dma);
+        spin_f(zeta, fz, dfz, ifdd);
+        const double ac = -ma, dac = -dma;
+        const double z3 = zeta * zeta * zeta, z4 = z3 * zeta, w4 = 1.0 - z4;
+        ec = e0 + ac * fz * ifdd * w4 + (e1 - e0) * fz * z4;
+        dec_rs = de0 + dac * fz * ifdd * w4 + (de1 - de0) * fz * z4;
+        dec_z = ac * ifdd * (dfz * w4 - 4.0 * z3 * fz) + (e1 - e0) * (dfz * z4 + 4.0 * z3 * fz);
+    }
     const double dec_r = dec_rs * drs;
-    const double dec_z = ac * ifdd * (dfz * w4 - 4.0 * z3 * fz) + (e1 - e0) * (dfz * z4 + 4.0 * z3 * fz);
     if (gt < 1e-36) {
         o.f = rho * ec;
         o.va = ec + rho * dec_r + (1.0 - zeta) * dec_z;
@@ -286,7 +298,7 @@ __device__ __forceinline__ XcOut pbe_correlation(double ra, double rb, double ga
     }
     const double b0 = beta / gam;
     const double zp = fmax(1.0 + zeta, 1e-24), zm = fmax(1.0 - zeta, 1e-24);
-    const double czp = cbrt(zp), czm = cbrt(zm);
+    const double czp = zeta != 0.0 ? cbrt(zp) : 1.0, czm = zeta != 0.0 ? cbrt(zm) : 1.0;
     const double zp23 = czp * czp, zm23 = czm * czm;
     const double phi = 0.5 * (zp23 + zm23), phi2 = phi * phi, phi3 = phi2 * phi;
     const double omz_dphi = ((1.0 - zeta) / czp - zm23) / 3.0;   // (1 - zeta) dphi/dzeta
