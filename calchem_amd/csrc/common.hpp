@@ -95,7 +95,6 @@ struct VPair {
 
 constexpr int kVTile = 64;       // tile of the tile-pair V kernel
 constexpr int kVPts = 16;        // points per V-stage step (MFMA k extent 4 x 4)
-constexpr int kVStage = 64;      // points (and wv) loaded into LDS per round trip in the V stage
 constexpr int kThreads = 256;    // 4 wavefronts of 64
 constexpr int kRhoPts = 16;      // points per rho-stage tile (MFMA N extent)
 constexpr int kGroupMaxTiles = 17;   // default 16-row tiles per register-resident V group (272 rows)

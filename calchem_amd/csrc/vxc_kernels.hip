@@ -568,11 +568,14 @@ __device__ __forceinline__ v4d lds_load4(const double* p) {
     return v4d{lo.x, lo.y, hi.x, hi.y};
 }
 
+// Point / wv slots of the V kernels (a ring of steps of 16 points).
+constexpr int kVSlots = 4;
+
 // LDS before the per-shell tables; the pair table is rounded up to keep
 // 16-byte alignment after it.
 template <typename T, int MAXT>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
-    return sizeof(double4) * 2 * kVStage + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+    return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
            sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * vgroup_ld<T>();
 }
 
@@ -585,12 +588,14 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     constexpr int LD = vgroup_ld<T>();
     static_assert((LD * sizeof(T)) % 16 == 0, "phi / Z rows must keep 16-byte alignment");
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
-    const int m = G.m, nsh = G.n_sh;
+    const int m = G.m, nsh = G.n_sh, npts = G.n_pts;
     const int T16 = (m + 15) >> 4, mp = T16 << 4;
     const int npairs = T16 * (T16 + 1) / 2;
+    const int nsteps = (npts + kVPts - 1) / kVPts;
     unsigned char* q = smem;
-    double4* sPtBuf = reinterpret_cast<double4*>(q);  // [kVStage points][kVStage wv]
-    q += sizeof(double4) * 2 * kVStage;
+    double4* sPt = reinterpret_cast<double4*>(q);  // [kVSlots][kVPts] points, then [kVSlots][kVPts] wv
+    double4* sWv = sPt + kVSlots * kVPts;
+    q += sizeof(double4) * 2 * kVSlots * kVPts;
     int2* sPair = reinterpret_cast<int2*>(q);
     q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
@@ -628,51 +633,51 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
 #pragma unroll
     for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
 
-    // Points and wv are staged kVStage at a time (one global round trip per
-    // kVStage / kVPts steps); slots past the group's last point get w = 0 and
-    // Z is zeroed there, so they add nothing.
-    for (int q0 = 0; q0 < G.n_pts; q0 += kVPts) {
-        const int qs = q0 % kVStage;
-        if (qs == 0) {
-            __syncthreads();
-            if (tid < 2 * kVStage) {
-                const int j = tid % kVStage, g = q0 + j;
-                double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
-                if (g < G.n_pts) {
-                    const int sg = G.pt_begin + g;
-                    if (tid < kVStage) v = a.pts[sg];
-                    else v = a.wv[a.wv_perm ? a.perm[sg] : sg];
-                }
-                sPtBuf[tid] = v;
+    // Two barriers per step of 16 points: between them the MFMA updates of
+    // step s run beside the radial sums of step s + 1 and the loads of the
+    // points and wv of step s + 2 (slot (s + 2) % kVSlots); then the AO
+    // values of step s + 1 replace those of step s.  Slots past the group's
+    // last point get w = 0 and Z is zeroed there, so they add nothing.
+    auto load_points = [&](int st) {
+        if (tid < 2 * kVPts) {
+            const int j = tid % kVPts, g = st * kVPts + j, slot = st % kVSlots;
+            double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (g < npts) {
+                const int sg = G.pt_begin + g;
+                v = tid < kVPts ? a.pts[sg] : a.wv[a.wv_perm ? a.perm[sg] : sg];
             }
+            (tid < kVPts ? sPt : sWv)[slot * kVPts + j] = v;
         }
-        __syncthreads();
-        const double4* sPt = sPtBuf + qs;
-        const double4* sWv = sPtBuf + kVStage + qs;
+    };
+    auto radial = [&](int st) {
+        const double4* pt = sPt + (st % kVSlots) * kVPts;
         for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
-            const double4 x = sPt[p];
-            const bool live = q0 + p < G.n_pts;
-            sRad[idx] = live ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
+            const double4 x = pt[p];
+            sRad[idx] = st * kVPts + p < npts ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
         }
-        __syncthreads();
+    };
+    auto evaluate = [&](int st) {
+        const double4* pt = sPt + (st % kVSlots) * kVPts;
+        const double4* wv = sWv + (st % kVSlots) * kVPts;
         for (int idx = tid; idx < mp * kVPts; idx += NT) {
             const int r = idx / kVPts, p = idx % kVPts;
             const RowL row = sRow[r];
             T v = 0, z = 0;
             if (row.sh >= 0) {
-                const double4 x = sPt[p];
-                const double4 w = sWv[p];
+                const double4 x = pt[p];
+                const double4 w = wv[p];
                 const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
                 v = f.v;
                 z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
             }
             sP[r * LD + p] = v;
-            sZ[r * LD + p] = q0 + p < G.n_pts ? z : T(0);
+            sZ[r * LD + p] = st * kVPts + p < npts ? z : T(0);
         }
-        __syncthreads();
-        // MFMA step k takes point 4 kq + k in lane (col, kq): A and B use the
-        // same point order, so the sum over the 16 points is unchanged.
+    };
+    // MFMA step k takes point 4 kq + k in lane (col, kq): A and B use the
+    // same point order, so the sum over the 16 points is unchanged.
+    auto update = [&]() {
         int cur_i = -1;
         V4 zi, pi;
 #pragma unroll
@@ -695,6 +700,24 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
                 acc[w] = M::op(pi[k], zj[k], acc[w]);
             }
         }
+    };
+
+    load_points(0);
+    if (nsteps > 1) load_points(1);
+    __syncthreads();
+    radial(0);
+    __syncthreads();
+    evaluate(0);
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+        update();
+        if (st + 1 < nsteps) radial(st + 1);
+        if (st + 2 < nsteps) load_points(st + 2);
+        __syncthreads();
+        if (st + 1 < nsteps) {
+            evaluate(st + 1);
+            __syncthreads();
+        }
     }
 #pragma unroll
     for (int w = 0; w < PPW; ++w) {
@@ -715,8 +738,6 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
 // interval and read in a later one, so the barrier between them is the only
 // ordering needed; waves that finish their MFMA share early go on with AO
 // evaluation, so MFMA and VALU work overlap on the SIMDs.
-constexpr int kVSlots = 4;
-
 template <int MAXT>
 __host__ __device__ constexpr size_t vgroup_pipe_lds_fixed() {
     return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +

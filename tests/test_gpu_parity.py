@@ -147,6 +147,30 @@ def test_fused_fp64_c8h18_block_shapes(block_points, group_points):
     assert np.abs(V - V.T).max() == 0.0
 
 
+@pytest.mark.parametrize("block_points,group_points,group_max_rows",
+                         [(256, 1024, 192), (32, 32, 0), (16, 48, 0)])
+def test_group_shapes_both_precisions(block_points, group_points, group_max_rows):
+    """192-row register-resident groups (the 12-tile kernels) and groups of one
+    to three 16-point steps (the pipelined mixed V kernel's prologue cases):
+    FP64 against the oracle, mixed against FP64 within its gate."""
+    mol, basis, P, w = _case("c8h18")
+    grid = MolecularGrid.explicit(mol, 40, 110)
+    eng = RksVxcEngine(basis, grid, block_points=block_points, group_points=group_points,
+                       group_max_rows=group_max_rows)
+    st = eng.stats()
+    e, V, ne = eng.evaluate_full("b3lyp", P)
+    e_ref, V_ref, ne_ref = O.vxc_rks(O.OracleBasis(basis), grid.points, "b3lyp", P)
+    dv = np.abs(V - V_ref).max()
+    e32, V32, n32 = eng.evaluate_full("b3lyp", P, "fp32")
+    dv32 = np.abs(V32 - V).max()
+    print(f"{block_points}/{group_points}/{group_max_rows}: groups={st['n_groups']} max_m={st['max_m_v']} "
+          f"dE={e - e_ref:.2e} dV={dv:.2e} mixed dE={e32 - e:.2e} dV={dv32:.2e}")
+    assert abs(e - e_ref) < E_TOL and dv < V_TOL and abs(ne - ne_ref) < N_TOL
+    # the gate is on E_xc; N (66 electrons) is held to 1e-6, a relative 1.5e-8
+    assert abs(e32 - e) < 1e-7 and dv32 < 1e-4 and abs(n32 - ne) < 1e-6
+    assert np.abs(V32 - V32.T).max() == 0.0
+
+
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_wide_blocks_use_tile_pair_kernel(prec):
     """The 20-water cluster in 4096-point blocks has blocks with more than 272
