@@ -83,7 +83,7 @@ struct Plan {
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, block_meta, rho_tiles, groups, group_rows,
-        group_ao,
+        group_meta, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
         asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
@@ -102,7 +102,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &block_meta, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
+                             &block_sh, &block_meta, &group_meta, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
@@ -516,6 +516,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     // goes to the tile-pair kernel.
     std::vector<VGroup> hg;
     std::vector<int2> hgrows, hgsh;  // hgsh: (global shell, group-local primitive offset)
+    std::vector<uint4> hgmeta;
     std::vector<int> hglist;
     std::vector<VPair> hpairs;
     long long scratch = 0;
@@ -558,9 +559,32 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                 hgrows.push_back(make_int2(mu, local[sidx]));
             }
             G.n_sh = static_cast<int>(hgsh.size()) - G.sh_begin;
+            G.meta = -1;
             if (m <= max_rows) {
                 G.tile_shift = 4;
                 G.n_tiles = (m + 15) / 16;
+                // packed records, copied into LDS as is by the group kernels
+                G.meta = static_cast<long long>(hgmeta.size());
+                for (int r = 0; r < G.n_tiles * 16; ++r) {
+                    RowL o{0.0, 0, -1};
+                    if (r < m) {
+                        const int2 e = hgrows[G.ao_begin + r];
+                        const DevAo& ao = hao[e.x];
+                        o.scale = ao.scale;
+                        o.lmn = ao.lx | (ao.ly << 2) | (ao.lz << 4);
+                        o.sh = e.y;
+                    }
+                    push_records(hgmeta, &o, 1);
+                }
+                for (int k = 0; k < G.n_sh; ++k) {
+                    DevShell d = hs[hgsh[G.sh_begin + k].x];
+                    d.prim_start = hgsh[G.sh_begin + k].y;
+                    push_records(hgmeta, &d, 1);
+                }
+                for (int k = 0; k < G.n_sh; ++k) {
+                    const DevShell& d = hs[hgsh[G.sh_begin + k].x];
+                    push_records(hgmeta, hp.data() + d.prim_start, d.n_prims);
+                }
                 hglist.push_back(gidx);
                 P.max_group_nsh = std::max(P.max_group_nsh, G.n_sh);
                 P.max_group_nprim = std::max(P.max_group_nprim, G.n_prim);
@@ -647,7 +671,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          upload(P.block_meta, hmeta) &&
          upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
          upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
-         upload(P.group_sh, hgsh) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
+         upload(P.group_sh, hgsh) && upload(P.group_meta, hgmeta) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_wg + 1)) &&
@@ -721,7 +745,8 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
     if (f32 && !P.v_fits_fp32) return kLimit;
     VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
-             P.group_sh.as<int2>(), wv, P.perm.as<int>(), wv_perm, P.scratch.as<double>()};
+             P.group_sh.as<int2>(), P.group_meta.as<uint4>(), wv, P.perm.as<int>(), wv_perm,
+             P.scratch.as<double>()};
     if (launch_vxc_group(va, f32, P.group_max_tiles, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) !=
         hipSuccess)
         return kLaunchFail;

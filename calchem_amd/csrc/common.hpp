@@ -85,6 +85,10 @@ struct VGroup {
     int sh_begin, n_sh;
     long long scratch_off;  // in doubles
     int n_prim, pad;
+    // Register-resident groups: offset, in 16-byte units, of the packed
+    // records RowL[16 n_tiles] (padding rows with sh = -1), DevShell[n_sh]
+    // (prim_start group-local), double2[n_prim]; -1 otherwise.
+    long long meta;
 };
 
 // One workgroup of the tile-pair V kernel: a 64x64 lower tile pair of a

@@ -123,6 +123,7 @@ struct VArgs {
     const int* group_list;  // register-resident kernel: groups it handles
     const int2* group_rows; // per group row: (global AO, group-local shell)
     const int2* group_sh;   // per group shell: (global shell, group-local primitive offset)
+    const uint4* gmeta;     // packed per-group records (VGroup::meta)
     const double4* wv;      // per sorted point, or per caller point when wv_perm
     const int* perm;
     int wv_perm;

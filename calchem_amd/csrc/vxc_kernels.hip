@@ -613,8 +613,13 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int col = lane & 15, kq = lane >> 4;
-    stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
-               sPrim, tid, NT);
+    {
+        // rows, then shells and primitives: two coalesced copies of the packed records
+        const uint4* __restrict__ src = a.gmeta + G.meta;
+        for (int i = tid; i < mp; i += NT) reinterpret_cast<uint4*>(sRow)[i] = src[i];
+        const int nrest = nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + G.n_prim;
+        for (int i = tid; i < nrest; i += NT) reinterpret_cast<uint4*>(sSh)[i] = src[mp + i];
+    }
 
     // Pairs p = ti (ti + 1) / 2 + tj (ti >= tj) in this order; wave w takes
     // the contiguous range [w c, (w + 1) c), c = ceil(npairs / NW), so
@@ -775,8 +780,13 @@ __global__ __launch_bounds__(1024) void k_vxc_group_pipe(VArgs a) {
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int col = lane & 15, kq = lane >> 4;
-    stage_meta(a.group_rows + G.ao_begin, m, mp, a.group_sh + G.sh_begin, nsh, a.shells, a.prims, a.aos, sRow, sSh,
-               sPrim, tid, NT);
+    {
+        // rows, then shells and primitives: two coalesced copies of the packed records
+        const uint4* __restrict__ src = a.gmeta + G.meta;
+        for (int i = tid; i < mp; i += NT) reinterpret_cast<uint4*>(sRow)[i] = src[i];
+        const int nrest = nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + G.n_prim;
+        for (int i = tid; i < nrest; i += NT) reinterpret_cast<uint4*>(sSh)[i] = src[mp + i];
+    }
     const int per_wave = (npairs + NW - 1) / NW;
     for (int p = tid; p < npairs; p += NT) {
         int ti = 0;
