@@ -78,6 +78,10 @@ struct Plan {
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
     int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, reduce_waves = 1;
+    // The register-resident group list is two launches: groups of more than
+    // kSmallGroupTiles tiles first (group_max_tiles kernel), then the rest
+    // (kSmallGroupTiles kernel), each costliest first.
+    int n_group_big = 0, max_small_nsh = 0, max_small_nprim = 0, max_big_nsh = 0, max_big_nprim = 0;
     int group_max_tiles = kGroupMaxTiles;
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
@@ -619,6 +623,15 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         };
         return cost(x) > cost(y);
     });
+    std::stable_partition(hglist.begin(), hglist.end(), [&](int g) { return hg[g].n_tiles > kSmallGroupTiles; });
+    for (int g : hglist) {
+        const bool big = hg[g].n_tiles > kSmallGroupTiles;
+        P.n_group_big += big;
+        int& nsh = big ? P.max_big_nsh : P.max_small_nsh;
+        int& npr = big ? P.max_big_nprim : P.max_small_nprim;
+        nsh = std::max(nsh, hg[g].n_sh);
+        npr = std::max(npr, hg[g].n_prim);
+    }
     // CSR: for each AO the groups holding it, ascending, with its row there.
     std::vector<int> hgb(n_ao + 1, 0);
     std::vector<RedEntry> hgg;
@@ -651,11 +664,14 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             P.class_fits[mode][c] = rho_lds_bytes(mode, P.classes[c]) <= P.lds_limit;
     }
 
-    if (P.n_group_list > 0 &&
-        vgroup_lds_bytes(false, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) > P.lds_limit)
+    const int n_small = P.n_group_list - P.n_group_big;
+    if ((P.n_group_big > 0 &&
+         vgroup_lds_bytes(false, P.group_max_tiles, P.max_big_nsh, P.max_big_nprim) > P.lds_limit) ||
+        (n_small > 0 && vgroup_lds_bytes(false, kSmallGroupTiles, P.max_small_nsh, P.max_small_nprim) > P.lds_limit))
         return kLimit;
-    P.v_fits_fp32 = P.n_group_list == 0 ||
-                    vgroup_lds_bytes(true, P.group_max_tiles, P.max_group_nsh, P.max_group_nprim) <= P.lds_limit;
+    P.v_fits_fp32 =
+        (P.n_group_big == 0 || vgroup_lds_bytes(true, P.group_max_tiles, P.max_big_nsh, P.max_big_nprim) <= P.lds_limit) &&
+        (n_small == 0 || vgroup_lds_bytes(true, kSmallGroupTiles, P.max_small_nsh, P.max_small_nprim) <= P.lds_limit);
     if (P.max_mpad_rho > 16384) return kLimit;  // the gather stages a block's AO indices in LDS
     P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
@@ -747,8 +763,12 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
              P.group_sh.as<int2>(), P.group_meta.as<uint4>(), wv, P.perm.as<int>(), wv_perm,
              P.scratch.as<double>()};
-    if (launch_vxc_group(va, f32, P.group_max_tiles, P.n_group_list, P.max_group_nsh, P.max_group_nprim, s) !=
-        hipSuccess)
+    if (launch_vxc_group(va, f32, P.group_max_tiles, P.n_group_big, P.max_big_nsh, P.max_big_nprim, s) != hipSuccess)
+        return kLaunchFail;
+    VArgs vs = va;
+    vs.group_list = va.group_list + P.n_group_big;
+    if (launch_vxc_group(vs, f32, kSmallGroupTiles, P.n_group_list - P.n_group_big, P.max_small_nsh,
+                         P.max_small_nprim, s) != hipSuccess)
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);

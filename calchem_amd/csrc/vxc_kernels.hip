@@ -533,6 +533,11 @@ struct VGroupCfg<double, 17> {
     using Rad = double2;
 };
 template <>
+struct VGroupCfg<double, 8> {
+    static constexpr int kWaves = 8;  // 5 FP64 pairs per wave; small LDS, two workgroups per CU
+    using Rad = double2;
+};
+template <>
 struct VGroupCfg<double, 12> {
     static constexpr int kWaves = 16;  // 5 FP64 pairs per wave: four waves per SIMD
     using Rad = double2;
@@ -749,11 +754,19 @@ __host__ __device__ constexpr size_t vgroup_pipe_lds_fixed() {
            sizeof(RowL) * MAXT * 16 + sizeof(float) * 4 * MAXT * 16 * vgroup_ld<float>();
 }
 
+// Waves of the pipelined kernel: 16 for groups of up to 12 or 17 tiles, 8
+// for the small-group class (up to 8 tiles), whose LDS lets two workgroups
+// share a CU.
 template <int MAXT>
-__global__ __launch_bounds__(1024) void k_vxc_group_pipe(VArgs a) {
+__host__ __device__ constexpr int vgroup_pipe_waves() {
+    return MAXT <= 8 ? 8 : 16;
+}
+
+template <int MAXT>
+__global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pipe(VArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     using T = float;
-    constexpr int NW = 16, NT = NW * 64, PPW = (vgroup_max_pairs<MAXT>() + NW - 1) / NW;
+    constexpr int NW = vgroup_pipe_waves<MAXT>(), NT = NW * 64, PPW = (vgroup_max_pairs<MAXT>() + NW - 1) / NW;
     constexpr int LD = vgroup_ld<T>();
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh, npts = G.n_pts;
@@ -1151,6 +1164,8 @@ static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim) {
 }
 
 size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim) {
+    if (max_tiles == 8)
+        return fp32 ? vgroup_lds_bytes_t<float, 8>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 8>(max_nsh, max_prim);
     if (max_tiles == 12)
         return fp32 ? vgroup_lds_bytes_t<float, 12>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 12>(max_nsh, max_prim);
     return fp32 ? vgroup_lds_bytes_t<float, 17>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 17>(max_nsh, max_prim);
@@ -1175,7 +1190,8 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
     if (n_groups <= 0) return hipSuccess;
     const size_t lds = vgroup_lds_bytes_t<T, MAXT>(max_nsh, max_prim);
     if constexpr (sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_vxc_group_pipe<MAXT>), dim3(n_groups), dim3(1024), lds, s, a);
+        hipLaunchKernelGGL((k_vxc_group_pipe<MAXT>), dim3(n_groups), dim3(vgroup_pipe_waves<MAXT>() * 64), lds, s,
+                           a);
     } else {
         constexpr int threads = VGroupCfg<T, MAXT>::kWaves * 64;
         hipLaunchKernelGGL((k_vxc_group<T, MAXT>), dim3(n_groups), dim3(threads), lds, s, a);
@@ -1185,6 +1201,9 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
 
 hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
                             hipStream_t s) {
+    if (max_tiles == 8)
+        return fp32 ? launch_vxc_group_t<float, 8>(a, n_groups, max_nsh, max_prim, s)
+                    : launch_vxc_group_t<double, 8>(a, n_groups, max_nsh, max_prim, s);
     if (max_tiles == 12)
         return fp32 ? launch_vxc_group_t<float, 12>(a, n_groups, max_nsh, max_prim, s)
                     : launch_vxc_group_t<double, 12>(a, n_groups, max_nsh, max_prim, s);
