@@ -93,17 +93,16 @@ __device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >
 // point); then wave w evaluates (phi, grad phi) for its 16-row tiles
 // w, w + 4, ... in exactly the lanes and slots where the MFMA result T of
 // those rows will land (lane (j, q), slot v: row Mma<TS>::row(q, v), point
-// j); phi goes to LDS as the MFMA B operand, (phi, grad phi) of the first
-// row tiles (3 in FP64, 2 in the mixed build, which holds FP64 leading-row
-// accumulators too) stay in registers for the epilogue (later row tiles
-// recompute them).  T = P_sub Phi on MFMA (density rows from L2, loaded two
+// j); phi goes to LDS as the MFMA B operand, (phi, grad phi) of the wave's
+// first two row tiles stay in registers for the epilogue (later row tiles,
+// in blocks of more than 128 rows, recompute them).  T = P_sub Phi on MFMA (density rows from L2, loaded two
 // chunks ahead), rho += phi T and grad rho += 2 grad phi T, summed over rows
 // with cross-lane shuffles and then over the waves.
 
 template <typename TS, bool HYB>
 __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = HYB ? 2 : 3;
+    constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = 2;
     const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
     if (t >= a.seg[xcd + 1]) return;
     const int2 tile = a.tiles[t];  // (block, first point within it)
