@@ -212,13 +212,26 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
 // the other atoms' terms of each point.
 constexpr int kBeckeAtomsPerLane = 4;  // up to 256 atoms
 
-__device__ __forceinline__ void becke_step_deriv(double nu, double& s, double& ds) {
+__device__ __forceinline__ void becke_f3_deriv(double nu, double& f3, double& ds) {
     const double x = fmin(fmax(nu, -1.0), 1.0);
     const double f1 = 0.5 * x * (3.0 - x * x), d1 = 1.5 * (1.0 - x * x);
     const double f2 = 0.5 * f1 * (3.0 - f1 * f1), d2 = 1.5 * (1.0 - f1 * f1);
-    const double f3 = 0.5 * f2 * (3.0 - f2 * f2), d3 = 1.5 * (1.0 - f2 * f2);
-    s = 0.5 * (1.0 - f3);
+    f3 = 0.5 * f2 * (3.0 - f2 * f2);
+    const double d3 = 1.5 * (1.0 - f2 * f2);
     ds = -0.5 * d3 * d2 * d1;
+}
+
+__device__ __forceinline__ void becke_step_deriv(double nu, double& s, double& ds) {
+    double f3;
+    becke_f3_deriv(nu, f3, ds);
+    s = 0.5 * (1.0 - f3);
+}
+
+// 1 / x to about an ulp: the hardware reciprocal and two Newton steps.
+__device__ __forceinline__ double recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
 }
 
 __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
@@ -300,22 +313,22 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                     const double mu = fmin(fmax((ra - sDist[b]) * inv, -1.0), 1.0);
                     const double asy = -a.asym[b * n + at];
                     const double nu = mu + asy * (1.0 - mu * mu);
-                    double s_ab, dsdnu;
-                    becke_step_deriv(nu, s_ab, dsdnu);
+                    double f3, dsdnu;
+                    becke_f3_deriv(nu, f3, dsdnu);
+                    const double s_ab = 0.5 * (1.0 - f3);
                     const double slope = dsdnu * (1.0 - 2.0 * asy * mu);
                     if (fabs(slope) < 1e-30) continue;
                     const double rv[3] = {a.xyz[3 * at] - a.xyz[3 * b], a.xyz[3 * at + 1] - a.xyz[3 * b + 1],
                                           a.xyz[3 * at + 2] - a.xyz[3 * b + 2]};
-                    // s_ba exactly as the cell loop formed it for atom b, so that it
-                    // is one of the factors counted there
-                    const double mu_b = fmin(fmax((sDist[b] - ra) * inv, -1.0), 1.0);
-                    double s_ba, ds_unused;
-                    becke_step_deriv(mu_b - asy * (1.0 - mu_b * mu_b), s_ba, ds_unused);
+                    // s_ba as the cell loop formed it for atom b (so that it is one
+                    // of the factors counted there): its nu is exactly -nu, f3 is
+                    // odd, so s_ba = (1 + f3) / 2 bit for bit
+                    const double s_ba = 0.5 * (1.0 + f3);
                     // products over the other pairs of at and of b: P / s when s is
                     // above 1e-12; else from the split products, in which s is one
                     // of the factors at or below 1e-12 (and one of the zeros if 0)
                     auto except = [&](int c, double sv) {
-                        if (sv > 1e-12) return sCell[c] / sv;
+                        if (sv > 1e-12) return sCell[c] * recip(sv);
                         if (sNTiny[c] == 1) return sBig[c];
                         if (sv > 0.0) return sNZero[c] > 0 ? 0.0 : sBig[c] * (sTiny[c] / sv);
                         return sNZero[c] > 1 ? 0.0 : sBig[c] * sTiny[c];
