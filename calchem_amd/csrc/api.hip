@@ -78,6 +78,7 @@ struct Plan {
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
     int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, reduce_waves = 1;
+    int reduce_segments = 1;  // workgroups per AO row in the V reduction (fewer rows than CUs x 8: more)
     // The register-resident group list is two launches: groups of more than
     // kSmallGroupTiles tiles first (group_max_tiles kernel), then the rest
     // (kSmallGroupTiles kernel), each costliest first.
@@ -87,7 +88,7 @@ struct Plan {
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
     DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, block_meta, rho_tiles, groups, group_rows,
-        group_meta, group_ao,
+        group_meta, red_part, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
         asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
@@ -106,7 +107,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &block_meta, &group_meta, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
+                             &block_sh, &block_meta, &group_meta, &red_part, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
@@ -675,6 +676,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (P.max_mpad_rho > 16384) return kLimit;  // the gather stages a block's AO indices in LDS
     P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
+    P.reduce_segments = std::max(1, std::min(4, 2048 / std::max(n_ao, 1)));
     const size_t need = sizeof(double) * (size_t)scratch + sizeof(RedEntry) * hgg.size() +
                         sizeof(double) * (size_t)P.p_elems * 2 +
                         sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
@@ -687,7 +689,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
          upload(P.block_meta, hmeta) &&
          upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
          upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
-         upload(P.group_sh, hgsh) && upload(P.group_meta, hgmeta) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
+         upload(P.group_sh, hgsh) && upload(P.group_meta, hgmeta) &&
+         (P.reduce_segments == 1 ||
+          P.red_part.alloc(sizeof(double) * (size_t)P.reduce_segments * n_ao * std::max(P.max_mol_ao, 1))) && upload(P.group_list, hglist) && upload(P.pairs, hpairs) &&
          upload(P.ao_gbegin, hgb) && upload(P.ao_groups, hgg) && P.scratch.alloc(sizeof(double) * (size_t)std::max<long long>(scratch, 1)) &&
          P.wv.alloc(sizeof(double4) * (size_t)std::max(P.n_pts, 1)) &&
          P.wv_caller.alloc(sizeof(double4) * (size_t)n_pts) && P.epart.alloc(sizeof(double) * (P.n_xc_wg + 1)) &&
@@ -773,7 +777,8 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);
     ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_ao.as<int>(), P.ao_mol.as<int>(),
-                  P.mols.as<MolInfo>(), P.n_ao, P.scratch.as<double>(), dV};
+                  P.mols.as<MolInfo>(), P.n_ao, P.scratch.as<double>(), dV, P.reduce_segments,
+                  std::max(P.max_mol_ao, 1), P.red_part.as<double>()};
     return launch_reduce_v(rd, P.reduce_waves, P.max_mol_ao, s) == hipSuccess ? kOk : kLaunchFail;
 }
 

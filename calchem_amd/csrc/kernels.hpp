@@ -160,6 +160,12 @@ struct ReduceArgs {
     int n_ao;               // global AO rows
     const double* scratch;
     double* V;              // packed per-molecule V blocks, full symmetric
+    // With n_seg > 1 each row's entries are split over n_seg workgroups
+    // (blockIdx.y), whose partial rows go to part[(seg n_ao + mu) row_stride
+    // + nu] and are summed in segment order by a second kernel.
+    int n_seg;
+    int row_stride;
+    double* part;
 };
 
 struct SigArgs {

@@ -926,12 +926,14 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
     for (int j = tid; j < nw * len; j += blockDim.x) srow[j] = 0.0;
     __syncthreads();
     double* __restrict__ my = srow + w * len;
-    // Wave w takes the w-th of nw contiguous runs of mu's entries, B at a
-    // time: the row indices and tile values of the B are requested together
-    // (one memory round trip), then added in entry order.
+    // Wave w of segment seg takes run seg nw + w of the n_seg nw contiguous
+    // runs of mu's entries, B at a time: the row indices and tile values of
+    // the B are requested together (one memory round trip), then added in
+    // entry order.
+    const int seg = blockIdx.y, runs = a.n_seg * nw;
     const int g0 = a.ao_gbegin[mu], e1 = a.ao_gbegin[mu + 1];
-    const int per = (e1 - g0 + nw - 1) / nw;
-    const int eb = g0 + w * per, ee = min(eb + per, e1);
+    const int per = (e1 - g0 + runs - 1) / runs;
+    const int eb = min(g0 + (seg * nw + w) * per, e1), ee = min(eb + per, e1);
     constexpr int B = 4;
     for (int e0 = eb; e0 < ee; e0 += B) {
         int li[B], ts[B], rows_at[B];
@@ -973,9 +975,30 @@ __global__ __launch_bounds__(1024) void k_reduce_v(ReduceArgs a) {
     __syncthreads();
     const long long n = mol.n_ao;
     double* __restrict__ V = a.V + mol.p_off;
+    double* __restrict__ part = a.part + (static_cast<long long>(seg) * a.n_ao + mu) * a.row_stride;
     for (int j = tid; j < len; j += blockDim.x) {
         double v = 0.0;
         for (int c = 0; c < nw; ++c) v += srow[c * len + j];
+        if (a.n_seg > 1) {
+            part[j] = v;
+        } else {
+            V[lmu * n + j] = v;
+            V[j * n + lmu] = v;
+        }
+    }
+}
+
+// Second pass of a segmented reduction: row mu of V is the sum of its
+// segments' partial rows in segment order, written and mirrored.
+__global__ __launch_bounds__(256) void k_reduce_v_combine(ReduceArgs a) {
+    const int mu = blockIdx.x;
+    const MolInfo mol = a.mols[a.ao_mol[mu]];
+    const int lmu = mu - mol.ao_base, len = lmu + 1;
+    const long long n = mol.n_ao;
+    double* __restrict__ V = a.V + mol.p_off;
+    for (int j = threadIdx.x; j < len; j += blockDim.x) {
+        double v = 0.0;
+        for (int sg = 0; sg < a.n_seg; ++sg) v += a.part[(static_cast<long long>(sg) * a.n_ao + mu) * a.row_stride + j];
         V[lmu * n + j] = v;
         V[j * n + lmu] = v;
     }
@@ -1230,8 +1253,9 @@ hipError_t launch_reduce_v(const ReduceArgs& a, int waves, int max_mol_ao, hipSt
         attr_set = 1;
     }
     // Row mu needs (local mu + 1) doubles per wave; size for the largest molecule.
-    hipLaunchKernelGGL(k_reduce_v, dim3(a.n_ao), dim3(64 * waves), sizeof(double) * (size_t)max_mol_ao * waves, s,
-                       a);
+    hipLaunchKernelGGL(k_reduce_v, dim3(a.n_ao, std::max(a.n_seg, 1)), dim3(64 * waves),
+                       sizeof(double) * (size_t)max_mol_ao * waves, s, a);
+    if (a.n_seg > 1) hipLaunchKernelGGL(k_reduce_v_combine, dim3(a.n_ao), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
