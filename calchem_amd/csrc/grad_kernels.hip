@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
 // a, a + 64, ...; per-atom sums stay in registers over the run and go to
 // this wave's partial row at the end.  The host atom gets minus the sum over
 // the other atoms' terms of each point.
-constexpr int kBeckeAtomsPerLane = 4;  // up to 256 atoms
+constexpr int kBeckeAtomsPerLane = 4;  // up to 256 atoms (APL = 1, 2 or 4 atoms per lane)
 
 __device__ __forceinline__ void becke_f3_deriv(double nu, double& f3, double& ds) {
     const double x = fmin(fmax(nu, -1.0), 1.0);
@@ -234,6 +234,7 @@ __device__ __forceinline__ double recip(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+template <int APL>
 __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int n = a.n_atoms, lane = threadIdx.x;
@@ -243,9 +244,9 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
     double* sTiny = sBig + n;                         // [n] product of the nonzero factors s_cd <= 1e-12
     int* sNTiny = reinterpret_cast<int*>(sTiny + n);  // [n] how many factors are <= 1e-12
     int* sNZero = sNTiny + n;                         // [n] how many factors are exactly 0
-    double acc[kBeckeAtomsPerLane][3];
+    double acc[APL][3];
 #pragma unroll
-    for (int k = 0; k < kBeckeAtomsPerLane; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+    for (int k = 0; k < APL; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
     const long long g0 = static_cast<long long>(blockIdx.x) * a.pts_per_wave;
     const long long g1 = min(g0 + a.pts_per_wave, static_cast<long long>(a.n_pts));
     for (long long g = g0; g < g1; ++g) {
@@ -298,7 +299,7 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
             const double wn = sCell[host] / zs, pref = raw * fxc / zs;
             double hsum[3] = {0.0, 0.0, 0.0};
 #pragma unroll
-            for (int k = 0; k < kBeckeAtomsPerLane; ++k) {
+            for (int k = 0; k < APL; ++k) {
                 const int at = lane + 64 * k;
                 if (at >= n || at == host) continue;
                 const double ra = sDist[at];
@@ -355,14 +356,14 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                 double t = hsum[c];
                 for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o);
 #pragma unroll
-                for (int k = 0; k < kBeckeAtomsPerLane; ++k)
+                for (int k = 0; k < APL; ++k)
                     if (lane + 64 * k == host) acc[k][c] -= t;
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int k = 0; k < kBeckeAtomsPerLane; ++k) {
+    for (int k = 0; k < APL; ++k) {
         const int at = lane + 64 * k;
         if (at < n)
             for (int c = 0; c < 3; ++c) a.partial[static_cast<long long>(blockIdx.x) * 3 * n + 3 * at + c] = acc[k][c];
@@ -415,8 +416,13 @@ hipError_t launch_grad_ao(const GradAoArgs& a, int n_tiles, size_t lds, hipStrea
 
 hipError_t launch_grad_becke(const GradBeckeArgs& a, int n_waves, hipStream_t s) {
     if (n_waves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_grad_becke, dim3(n_waves), dim3(64), (sizeof(double) * 4 + sizeof(int) * 2) * (size_t)a.n_atoms,
-                       s, a);
+    const size_t lds = (sizeof(double) * 4 + sizeof(int) * 2) * (size_t)a.n_atoms;
+    if (a.n_atoms <= 64)
+        hipLaunchKernelGGL(k_grad_becke<1>, dim3(n_waves), dim3(64), lds, s, a);
+    else if (a.n_atoms <= 128)
+        hipLaunchKernelGGL(k_grad_becke<2>, dim3(n_waves), dim3(64), lds, s, a);
+    else
+        hipLaunchKernelGGL(k_grad_becke<kBeckeAtomsPerLane>, dim3(n_waves), dim3(64), lds, s, a);
     return hipGetLastError();
 }
 
