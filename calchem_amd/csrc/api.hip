@@ -1215,7 +1215,10 @@ int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double* Pa, const
     const bool uks = Pb != nullptr;
     const size_t nn = (size_t)P.n_ao * P.n_ao;
     const long long n_tiles = (long long)P.n_rho_tiles, passes = uks ? 2 : 1;
-    const int pts_per_wave = 64;  // Becke-derivative points per wave
+    // Becke-derivative points per wave: enough waves to fill the GPU several
+    // times over (up to 65536 of them, which bounds the partial rows)
+    int pts_per_wave = 16;
+    while (pts_per_wave < 256 && (P.n_pts + pts_per_wave - 1) / pts_per_wave > 65536) pts_per_wave *= 2;
     const long long n_waves = (P.n_pts + pts_per_wave - 1) / pts_per_wave;
     const long long rows = n_tiles * passes + n_waves;
     const long long per_chunk = 512, chunks = (rows + per_chunk - 1) / per_chunk;
