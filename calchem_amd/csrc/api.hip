@@ -99,10 +99,21 @@ struct Plan {
     std::mutex mu;
     Profile prof;
     std::vector<hipGraphExec_t> graphs;  // captured builds; id = index + 1, null once destroyed
+    // Side streams and events that let the independent launches of the rho
+    // stage (its LDS classes) run side by side: the caller's stream forks to
+    // them after the gather and joins them before the functional.
+    static constexpr int kSide = 3;
+    hipStream_t side[kSide] = {};
+    hipEvent_t fork_ev = nullptr, join_ev[kSide] = {};
     ~Plan() {
         for (hipGraphExec_t g : graphs)
             if (g) hipGraphExecDestroy(g);
         for (hipEvent_t e : prof.ev) hipEventDestroy(e);
+        for (int i = 0; i < kSide; ++i) {
+            if (join_ev[i]) hipEventDestroy(join_ev[i]);
+            if (side[i]) hipStreamDestroy(side[i]);
+        }
+        if (fork_ev) hipEventDestroy(fork_ev);
         if (stream) hipStreamDestroy(stream);
     }
     size_t device_bytes() const {
@@ -707,6 +718,11 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     // kernel: their sorted rho stays zero for the life of the plan.
     if (hipMemset(P.rho_s.p, 0, P.rho_s.bytes) != hipSuccess) return kLaunchFail;
     if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) return kNoDevice;
+    for (int i = 0; i < Plan::kSide; ++i)
+        if (hipStreamCreateWithFlags(&P.side[i], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&P.join_ev[i], hipEventDisableTiming) != hipSuccess)
+            return kNoDevice;
+    if (hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming) != hipSuccess) return kNoDevice;
     if (hipDeviceSynchronize() != hipSuccess) return kLaunchFail;
     std::lock_guard<std::mutex> lk(g_table_mu);
     g_table[id] = plan;
@@ -741,6 +757,26 @@ void resolve_profile(Plan& P) {
 // LDS budget.  rho goes to `sorted` per sorted point, or, when sorted is
 // null, to P.rho per caller point (points dropped at plan time keep what
 // P.rho held).  With gather_mark the profiling mark after the gather is set.
+// Runs fn(i, stream) for i in [0, n): i = 0 on s, the others on the plan's
+// side streams, forked from s after what s has queued so far; s then waits
+// for all of them.  The launches must be independent of each other.
+template <class F>
+int fan_out(Plan& P, hipStream_t s, int n, F fn) {
+    if (n <= 1) return n == 1 ? fn(0, s) : kOk;
+    if (hipEventRecord(P.fork_ev, s) != hipSuccess) return kLaunchFail;
+    const int used = std::min(n - 1, Plan::kSide);
+    for (int j = 0; j < used; ++j)
+        if (hipStreamWaitEvent(P.side[j], P.fork_ev, 0) != hipSuccess) return kLaunchFail;
+    for (int i = 0; i < n; ++i) {
+        const int rc = fn(i, i == 0 ? s : P.side[(i - 1) % Plan::kSide]);
+        if (rc != kOk) return rc;
+    }
+    for (int j = 0; j < used; ++j)
+        if (hipEventRecord(P.join_ev[j], P.side[j]) != hipSuccess || hipStreamWaitEvent(s, P.join_ev[j], 0) != hipSuccess)
+            return kLaunchFail;
+    return kOk;
+}
+
 int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark) {
     GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
@@ -748,14 +784,16 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
     if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
-    for (size_t c = 0; c < P.classes.size(); ++c) {
+    for (size_t c = 0; c < P.classes.size(); ++c)
         if (!P.class_fits[mode][c]) return kLimit;
+    // the LDS classes cover disjoint tiles: side by side (their tails overlap)
+    return fan_out(P, s, static_cast<int>(P.classes.size()), [&](int c, hipStream_t st) {
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
                    P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
-                   P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(), sorted ? 0 : 1};
-        if (launch_rho(ra, mode, P.classes[c], s) != hipSuccess) return kLaunchFail;
-    }
-    return kOk;
+                   P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(),
+                   sorted ? 0 : 1};
+        return launch_rho(ra, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
+    });
 }
 
 // V stage: register-resident kernel over its groups, tile-pair kernel over
