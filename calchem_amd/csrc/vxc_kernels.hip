@@ -10,9 +10,12 @@
 //   k_vxc_group  per group (up to 272 AO rows): every lower 16x16 tile pair
 //             of the group's V in registers for the whole pass over its
 //             points; Z = w v_rho/2 Phi + wu . dPhi and Phi staged in LDS 16
-//             points at a time, acc += Z_I Phi_J^T + Phi_I Z_J^T on MFMA.
+//             points at a time, acc += Z_I Phi_J^T + Phi_I Z_J^T on MFMA
+//             (FP64); k_vxc_group_pipe is its pipelined FP32 form for the
+//             mixed build.  Both come in a 17-tile and an 8-tile class.
 //   k_vxc     the same per 64x64 tile pair, for groups too wide for the above.
-//   k_reduce_v   deterministic sum of the group tiles into the full symmetric V.
+//   k_reduce_v   deterministic sum of the group tiles into the full symmetric V
+//             (k_reduce_v_combine adds the partial rows of a split row).
 //   k_reduce_en  deterministic sum of the E_xc / N partials.
 // T = double runs FP64 MFMA (v_mfma_f64_16x16x4f64); T = float runs FP32 MFMA
 // (v_mfma_f32_16x16x4f32) with FP32 accumulation inside a tile and FP64 for
