@@ -268,7 +268,7 @@ def main() -> int:
     from calchem_amd.basis import BasisSet
     from calchem_amd.grid import MolecularGrid
     from calchem_amd.overlap import overlap_matrix
-    from calchem_amd.vxc import RksVxcEngine
+    from calchem_amd.vxc import RksVxcEngine, UksVxcEngine
 
     if args.workload == "batch64":
         return bench_batched(args)
@@ -327,6 +327,30 @@ def main() -> int:
         G = eng.xc_gradient(func, P)
     grad_ms = (time.perf_counter() - t0) / 3 * 1e3
     log(f"XC nuclear gradient: {grad_ms:.2f} ms, |sum over atoms| {np.abs(G.sum(axis=0)).max():.1e}")
+
+    # Unrestricted build of the same workload (100 alpha / 99 beta orbitals), device buffers.
+    uks = UksVxcEngine(basis, grid, args.block_points, args.group_points,
+                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows)
+    Pa = 0.5 * P
+    Pb = 0.5 * configs.random_density(basis, w["n_occ"] - 1, w["seed"] + 1, S)
+    dPa, dPb = hr.DeviceArray.from_numpy(Pa), hr.DeviceArray.from_numpy(Pb)
+    dVb = hr.DeviceArray(n * n * 8)
+    uks_ms = {}
+    for prec in ("fp64", "fp32"):
+        for _ in range(2):
+            uks.build_device_uks(func, dPa.address, dPb.address, dV.address, dVb.address, den.address,
+                                 stream.address, prec)
+        stream.synchronize()
+        ev0, ev1 = hr.Event(), hr.Event()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            uks.build_device_uks(func, dPa.address, dPb.address, dV.address, dVb.address, den.address,
+                                 stream.address, prec)
+        ev1.record(stream)
+        ev1.synchronize()
+        uks_ms[prec] = ev0.elapsed_ms(ev1) / args.steps
+    uks.close()
+    log(f"UKS builds: {uks_ms['fp64']:.3f} ms FP64, {uks_ms['fp32']:.3f} ms mixed")
 
     # Host-buffer path (P copied in, V copied out over PCIe, synchronous), for reference only.
     pcie = {}
@@ -404,6 +428,7 @@ def main() -> int:
         "mixed_vs_fp64_speedup": r64["ms"] / r32["ms"],
         "pcie_inclusive_builds_per_s": pcie,
         "xc_gradient_ms": grad_ms,
+        "uks_ms_per_build": uks_ms,
         "mixed_errors_vs_fp64_gpu": {"dE_xc": dE, "max_dV": dV32, "dN": dN, "gate_1e-7": gate},
         "fp64_gpu_vs_oracle_sample": parity,
         "build_tflops": build_tflops,
