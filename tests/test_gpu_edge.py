@@ -1,0 +1,63 @@
+"""Edge cases the reference's own engine tests hold, on the GPU through the C ABI.
+
+Reference checks followed: a grid of one zero-weight point gives E_xc = 0 and
+V = 0 exactly for both the restricted and the unrestricted engine, and finite
+symmetric results for an ordinary coarse grid
+(crates/calchem-dft/src/vxc.rs:2745-2783).  Also: a zero density matrix
+(every point below the density screen) gives E_xc = N = 0 and V = 0 exactly.
+"""
+import numpy as np
+import pytest
+
+from calchem_amd import configs
+from calchem_amd.basis import BasisSet
+from calchem_amd.grid import MolecularGrid
+from calchem_amd.molecule import Atom, Molecule
+from calchem_amd.vxc import RksVxcEngine, UksVxcEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def _water():
+    # crates/calchem-dft/src/vxc.rs:2345-2362
+    return Molecule([Atom.from_angstrom("O", (0.02, -0.04, 0.11)), Atom.from_angstrom("H", (0.81, 0.59, -0.37)),
+                     Atom.from_angstrom("H", (-0.74, 0.63, -0.42))])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_zero_weight_grid_gives_exact_zero(prec):
+    mol = _water()
+    basis = BasisSet.from_name("sto-3g", mol)
+    Pa = 0.5 * configs.random_density(basis, 3, 1)
+    Pb = 0.5 * configs.random_density(basis, 2, 2)
+    grid = MolecularGrid(np.array([[0.0, 0.0, 0.0, 0.0, 0.0, 0.0]]), "coarse")
+    e, V, ne = RksVxcEngine(basis, grid).evaluate_full("b3lyp", Pa + Pb, prec)
+    assert e == 0.0 and ne == 0.0 and np.all(V == 0.0)
+    eu, Va, Vb, nu = UksVxcEngine(basis, grid).evaluate_full("b3lyp", Pa, Pb, prec)
+    assert eu == 0.0 and nu == 0.0 and np.all(Va == 0.0) and np.all(Vb == 0.0)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_coarse_grid_finite_and_symmetric(prec):
+    mol = _water()
+    basis = BasisSet.from_name("sto-3g", mol)
+    Pa = 0.5 * configs.random_density(basis, 3, 1)
+    Pb = 0.5 * configs.random_density(basis, 2, 2)
+    grid = MolecularGrid.build(mol, "coarse")
+    e, V, ne = RksVxcEngine(basis, grid).evaluate_full("b3lyp", Pa + Pb, prec)
+    eu, Va, Vb, nu = UksVxcEngine(basis, grid).evaluate_full("b3lyp", Pa, Pb, prec)
+    assert np.isfinite(e) and np.isfinite(eu) and np.isfinite(ne) and np.isfinite(nu)
+    for M in (V, Va, Vb):
+        assert np.all(np.isfinite(M)) and np.abs(M - M.T).max() == 0.0
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_zero_density_gives_exact_zero(prec):
+    mol = _water()
+    basis = BasisSet.from_name("6-31g(d)", mol)
+    grid = MolecularGrid.build(mol, "coarse")
+    Z = np.zeros((basis.n_ao, basis.n_ao))
+    e, V, ne = RksVxcEngine(basis, grid).evaluate_full("pbe", Z, prec)
+    assert e == 0.0 and ne == 0.0 and np.all(V == 0.0)
+    eu, Va, Vb, nu = UksVxcEngine(basis, grid).evaluate_full("pbe", Z, Z, prec)
+    assert eu == 0.0 and nu == 0.0 and np.all(Va == 0.0) and np.all(Vb == 0.0)
