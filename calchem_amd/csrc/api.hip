@@ -77,7 +77,7 @@ struct Plan {
     double flops_rho = 0, flops_v = 0, flops_rho_pad = 0, flops_v_pad = 0, flops_rho_fp64_rows = 0;
     double mixed_threshold = 0.25;
     size_t lds_limit = 0;
-    int n_group_list = 0, max_group_nsh = 0, max_group_nprim = 0, reduce_waves = 1;
+    int n_group_list = 0, reduce_waves = 1;
     int reduce_segments = 1;  // workgroups per AO row in the V reduction (fewer rows than CUs x 8: more)
     // The register-resident group list is two launches: groups of more than
     // kSmallGroupTiles tiles first (group_max_tiles kernel), then the rest
@@ -602,8 +602,6 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                     push_records(hgmeta, hp.data() + d.prim_start, d.n_prims);
                 }
                 hglist.push_back(gidx);
-                P.max_group_nsh = std::max(P.max_group_nsh, G.n_sh);
-                P.max_group_nprim = std::max(P.max_group_nprim, G.n_prim);
                 P.flops_v_pad += 2.0 * G.n_tiles * (G.n_tiles + 1) * 256.0 * (double)round_up(npt, kVPts);
             } else {
                 G.tile_shift = 6;
