@@ -87,7 +87,8 @@ struct Plan {
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
     std::vector<MolInfo> mol_host;
-    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, block_meta, rho_tiles, groups, group_rows,
+    DevBuf shells, prims, aos, pts, perm, blocks, block_rows, block_sh, block_meta, rho_tiles, grad_tiles, groups,
+        group_rows,
         group_meta, red_part, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
@@ -118,7 +119,7 @@ struct Plan {
     }
     size_t device_bytes() const {
         const DevBuf* b[] = {&shells,   &prims,     &aos,      &pts,   &perm,      &blocks,   &block_rows,
-                             &block_sh, &block_meta, &group_meta, &red_part, &rho_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
+                             &block_sh, &block_meta, &group_meta, &red_part, &rho_tiles, &grad_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
@@ -481,7 +482,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     // the segment with the least cost so far), each segment ordered costliest
     // block first; a block's tiles stay consecutive.  Cost of a tile: m_pad^2
     // plus a fixed part for its staging and AO work (that of m_pad = 128).
-    std::vector<int2> htiles;
+    std::vector<int2> htiles, hgtiles;  // rho workgroups; 16-point tiles of the gradient kernel
     {
         const int ncls = std::max((P.max_mpad_rho + 63) / 64, 1);
         std::vector<std::vector<int>> bycls(ncls);
@@ -503,6 +504,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         auto block_cost = [&](int b) {
             return (double)((hb[b].n_pts + kRhoPts - 1) / kRhoPts) * ((double)hb[b].m_pad * hb[b].m_pad + 16384.0);
         };
+        const int step = kRhoSub * kRhoPts;  // points per rho workgroup
         for (int c = 0; c < ncls; ++c) {
             if (bycls[c].empty()) continue;
             std::vector<int>& bl = bycls[c];
@@ -518,8 +520,13 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             for (int x = 0; x < kXcds; ++x) {
                 cls[c].seg[x] = static_cast<int>(htiles.size()) - cls[c].tile_begin;
                 for (int b : seg[x])
-                    for (int o = 0; o < hb[b].n_pts; o += kRhoPts) htiles.push_back(make_int2(b, o));
+                    for (int o = 0; o < hb[b].n_pts; o += step) htiles.push_back(make_int2(b, o));
             }
+            // the XC gradient's AO kernel keeps one workgroup per 16 points
+            cls[c].grad_tile_begin = static_cast<int>(hgtiles.size());
+            for (int b : bl)
+                for (int o = 0; o < hb[b].n_pts; o += kRhoPts) hgtiles.push_back(make_int2(b, o));
+            cls[c].grad_n_tiles = static_cast<int>(hgtiles.size()) - cls[c].grad_tile_begin;
             cls[c].seg[kXcds] = static_cast<int>(htiles.size()) - cls[c].tile_begin;
             cls[c].n_tiles = cls[c].seg[kXcds];
             P.classes.push_back(cls[c]);
@@ -696,7 +703,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
          upload(P.block_meta, hmeta) &&
-         upload(P.rho_tiles, htiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
+         upload(P.rho_tiles, htiles) && upload(P.grad_tiles, hgtiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
          upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_meta, hgmeta) &&
          (P.reduce_segments == 1 ||
@@ -1294,11 +1301,11 @@ int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double* Pa, const
         for (size_t c = 0; c < P.classes.size(); ++c) {
             const RhoClass& cl = P.classes[c];
             GradAoArgs g{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
-                         P.rho_tiles.as<int2>() + cl.tile_begin, P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(),
+                         P.grad_tiles.as<int2>() + cl.grad_tile_begin, P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(),
                          P.block_sh.as<int2>(), P.shell_atom.as<int>(), P.psub64.as<double>(),
                          pass == 0 ? P.gwx_a.as<double4>() : P.gwx_b.as<double4>(), P.host_s.as<int>(), na, -2.0,
-                         P.gpart.as<double>() + (size_t)(pass * n_tiles + cl.tile_begin) * width};
-            if (launch_grad_ao(g, cl.n_tiles, grad_ao_lds_bytes(cl.max_mpad, cl.max_nsh, cl.max_prim), s) !=
+                         P.gpart.as<double>() + (size_t)(pass * n_tiles + cl.grad_tile_begin) * width};
+            if (launch_grad_ao(g, cl.grad_n_tiles, grad_ao_lds_bytes(cl.max_mpad, cl.max_nsh, cl.max_prim), s) !=
                 hipSuccess)
                 return kLaunchFail;
         }

@@ -101,6 +101,7 @@ constexpr int kVTile = 64;       // tile of the tile-pair V kernel
 constexpr int kVPts = 16;        // points per V-stage step (MFMA k extent 4 x 4)
 constexpr int kThreads = 256;    // 4 wavefronts of 64
 constexpr int kRhoPts = 16;      // points per rho-stage tile (MFMA N extent)
+constexpr int kRhoSub = 2;       // 16-point tiles one rho workgroup takes in turn (same block)
 constexpr int kSmallGroupTiles = 8;  // V groups of at most this many 16-row tiles use the small-group kernels
 constexpr int kGroupMaxTiles = 17;   // default 16-row tiles per register-resident V group (272 rows)
 // Elements after the last gathered density block: the rho kernel's look-ahead

@@ -108,7 +108,8 @@ void becke_pair_tables(int n_atoms, const int* z, const double* xyz, double* inv
 // about equal cost, segment x = [seg[x], seg[x + 1]) (class-local), each a
 // run of whole blocks, costliest first; XCD x walks segment x.
 struct RhoClass {
-    int tile_begin, n_tiles;  // range in the class-ordered tile list
+    int tile_begin, n_tiles;  // range in the class-ordered list of rho workgroups (kRhoSub x 16 points)
+    int grad_tile_begin, grad_n_tiles;  // range in the 16-point tile list of the XC gradient's AO kernel
     int max_mpad, max_nsh, max_m64, max_prim;
     int seg[kXcds + 1];
 };

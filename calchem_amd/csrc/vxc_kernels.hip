@@ -78,7 +78,7 @@ struct Vec4<float> {
 // rows, shells, primitives (the block's packed records, copied as one run),
 // FP64 leading-row values, row values.
 __host__ __device__ constexpr size_t rho_lds_fixed() {
-    return sizeof(double4) * kRhoPts + sizeof(double) * 16 * kRhoPts;
+    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * kRhoPts;
 }
 
 // Workgroups are dispatched round robin over the XCDs (workgroup i to XCD
@@ -108,13 +108,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = 2;
     const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
     if (t >= a.seg[xcd + 1]) return;
-    const int2 tile = a.tiles[t];  // (block, first point within it)
+    const int2 tile = a.tiles[t];  // (block, first point within it): up to kRhoSub x 16 points
     const RhoBlock blk = a.blocks[tile.x];
     const int m = blk.m, mp = blk.m_pad, nsh = blk.n_sh, T16 = blk.m_pad >> 4;
     const int m64 = HYB ? blk.m64 : 0;
     unsigned char* q = smem;
-    double4* sPt = reinterpret_cast<double4*>(q);
-    q += sizeof(double4) * kRhoPts;
+    double4* sPtAll = reinterpret_cast<double4*>(q);  // [kRhoSub * PTS]
+    q += sizeof(double4) * kRhoSub * kRhoPts;
     double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 sums][PTS]
     q += sizeof(double) * 16 * PTS;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
@@ -140,16 +140,20 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     // are the ones contract() makes for that tile; see there for the tail).
     V4 pre0{}, pre1{};
     double pre64[4] = {0.0, 0.0, 0.0, 0.0};
-    if (wave < T16) {
-        const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>((wave << 4) + col) * mp;
-        pre0 = *reinterpret_cast<const V4*>(Arow + m64 + 4 * kq);
-        pre1 = *reinterpret_cast<const V4*>(Arow + m64 + 16 + 4 * kq);
-        if (HYB && m64 >= 16) {
-            const double* __restrict__ A64 = a.P64 + blk.psub + static_cast<long long>((wave << 4) + f64_row_as_f32(col)) * mp;
+    auto preload = [&]() {
+        if (wave < T16) {
+            const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>((wave << 4) + col) * mp;
+            pre0 = *reinterpret_cast<const V4*>(Arow + m64 + 4 * kq);
+            pre1 = *reinterpret_cast<const V4*>(Arow + m64 + 16 + 4 * kq);
+            if (HYB && m64 >= 16) {
+                const double* __restrict__ A64 =
+                    a.P64 + blk.psub + static_cast<long long>((wave << 4) + f64_row_as_f32(col)) * mp;
 #pragma unroll
-            for (int s = 0; s < 4; ++s) pre64[s] = A64[4 * s + kq];
+                for (int s = 0; s < 4; ++s) pre64[s] = A64[4 * s + kq];
+            }
         }
-    }
+    };
+    preload();
     {
         // rows, shells and primitives: one coalesced copy of the packed records
         const uint4* __restrict__ src = a.meta + blk.meta;
@@ -157,10 +161,19 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         const int n16 = mp + nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + blk.n_prim;
         for (int i = tid; i < n16; i += kThreads) dst[i] = src[i];
     }
-    const int t0 = tile.y, np = min(PTS, blk.n_pts - t0);
-    if (tid < PTS) sPt[tid] = tid < np ? a.pts[blk.pt_begin + t0 + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
+    // The workgroup's points: kRhoSub sub-tiles of 16, taken one after the
+    // other with the block's records staged once.
+    const int t0 = tile.y, np_all = min(kRhoSub * PTS, blk.n_pts - t0);
+    if (tid < kRhoSub * PTS)
+        sPtAll[tid] = tid < np_all ? a.pts[blk.pt_begin + t0 + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
-    {
+    // (each sub-tile is a separate inlined copy of the body, not a loop:
+    // a loop lets the compiler hoist per-tile addresses across both passes
+    // and nearly doubles the registers)
+    auto process = [&](const int sub) {
+        const double4* sPt = sPtAll + sub * PTS;
+        const int np = min(PTS, np_all - sub * PTS), tsub = t0 + sub * PTS;
+        if (sub > 0) preload();
 
         // Radial sums once per (shell, point).
         for (int idx = tid; idx < nsh * PTS; idx += kThreads) {
@@ -330,9 +343,15 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
                 gy += sRed[(w * 4 + 2) * PTS + tid];
                 gz += sRed[(w * 4 + 3) * PTS + tid];
             }
-            const int g = blk.pt_begin + t0 + tid;
+            const int g = blk.pt_begin + tsub + tid;
             a.rho_out[a.rho_perm ? a.perm[g] : g] = make_double4(rho, 2.0 * gx, 2.0 * gy, 2.0 * gz);
         }
+    };
+    static_assert(kRhoSub == 2, "two sub-tiles per workgroup");
+    process(0);
+    if (np_all > PTS) {
+        __syncthreads();  // the second sub-tile rewrites the radial table, phi and the sums
+        process(1);
     }
 }
 
