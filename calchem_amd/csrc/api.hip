@@ -199,6 +199,34 @@ void push_records(std::vector<uint4>& out, const T* rec, int n) {
     if (n > 0) std::memcpy(out.data() + at, rec, sizeof(T) * (size_t)n);
 }
 
+// Block- or group-local shell list of an ordered row list: every shell of
+// the rows once, those with more primitives first (ties in order of first
+// use), so that the (shell, point) lanes of one wave's radial sums loop over
+// equally many primitives.  Appends (global shell, local primitive offset) to
+// shl and (AO, local shell) to out_rows; returns the list's primitive count.
+// `local` is n_shells entries of -1 on entry and on return.
+int append_shell_list(const std::vector<int>& rows, const std::vector<DevAo>& hao, const std::vector<DevShell>& hs,
+                      std::vector<int>& local, std::vector<int2>& shl, std::vector<int2>& out_rows) {
+    std::vector<int> order;
+    for (int mu : rows) {
+        const int s = hao[mu].shell;
+        if (local[s] < 0) {
+            local[s] = static_cast<int>(order.size());
+            order.push_back(s);
+        }
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hs[x].n_prims > hs[y].n_prims; });
+    int n_prim = 0;
+    for (int k = 0; k < static_cast<int>(order.size()); ++k) {
+        local[order[k]] = k;
+        shl.push_back(make_int2(order[k], n_prim));
+        n_prim += hs[order[k]].n_prims;
+    }
+    for (int mu : rows) out_rows.push_back(make_int2(mu, local[hao[mu].shell]));
+    for (int x : order) local[x] = -1;
+    return n_prim;
+}
+
 // One molecule of a plan as the caller hands it in: its shells
 // [shell_begin, shell_end) (ao_offset local to the molecule), its points
 // [pt_begin, pt_end) of the caller's point array, its AO count.
@@ -415,6 +443,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     std::vector<int2> hrows;
     std::vector<int2> hsh;  // (global shell, block-local primitive offset)
     std::vector<std::vector<int>> blist(P.n_blocks);
+    std::vector<int> local_sh(n_shells, -1);
     for (int b = 0; b < P.n_blocks; ++b) {
         std::vector<int> big, small;
         for (int k = cand_begin[b]; k < cand_begin[b + 1]; ++k) {
@@ -434,16 +463,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         B.m_pad = mp;
         B.m64 = std::min(round_up(static_cast<int>(big.size()), 16), mp);  // whole 16-column chunks
         B.sh_begin = static_cast<int>(hsh.size());
-        std::vector<int> local(n_shells, -1);
-        for (int mu : rows) {
-            const int s = hao[mu].shell;
-            if (local[s] < 0) {
-                local[s] = static_cast<int>(hsh.size()) - B.sh_begin;
-                hsh.push_back(make_int2(s, B.n_prim));
-                B.n_prim += hs[s].n_prims;
-            }
-            hrows.push_back(make_int2(mu, local[s]));
-        }
+        B.n_prim = append_shell_list(rows, hao, hs, local_sh, hsh, hrows);
         B.n_sh = static_cast<int>(hsh.size()) - B.sh_begin;
         hrows.resize(hrows.size() + (mp - m), make_int2(-1, 0));
         // Packed records, the layout the rho kernel copies into LDS as is.
@@ -571,16 +591,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             G.n_pts = npt;
             G.scratch_off = scratch;
             G.sh_begin = static_cast<int>(hgsh.size());
-            std::vector<int> local(n_shells, -1);
-            for (int mu : u) {
-                const int sidx = hao[mu].shell;
-                if (local[sidx] < 0) {
-                    local[sidx] = static_cast<int>(hgsh.size()) - G.sh_begin;
-                    hgsh.push_back(make_int2(sidx, G.n_prim));
-                    G.n_prim += hs[sidx].n_prims;
-                }
-                hgrows.push_back(make_int2(mu, local[sidx]));
-            }
+            G.n_prim = append_shell_list(u, hao, hs, local_sh, hgsh, hgrows);
             G.n_sh = static_cast<int>(hgsh.size()) - G.sh_begin;
             G.meta = -1;
             if (m <= max_rows) {
