@@ -24,6 +24,14 @@ struct AoVal {
     double v, gx, gy, gz;
 };
 
+// x^l and d/dx x^l for l <= 3 in straight-line code (selects, no branches):
+// q = x^(l-1) for l >= 1, p = x q (1 for l = 0), d = l q.
+__device__ __forceinline__ void mono_sel(double x, int l, double& p, double& d) {
+    const double q = (l & 2) ? ((l & 1) ? x * x : x) : 1.0;
+    p = l ? x * q : 1.0;
+    d = static_cast<double>(l) * q;
+}
+
 __device__ __forceinline__ AoVal ao_eval(const DevShell* __restrict__ shells, const double2* __restrict__ prims,
                                          const DevAo& ao, double x, double y, double z) {
     AoVal o{0.0, 0.0, 0.0, 0.0};
@@ -78,28 +86,25 @@ __device__ __forceinline__ double2 shell_radial(const DevShell& s, const double2
     return make_double2(r0, r1);
 }
 
+// Straight-line: the unscaled values u are formed first and the drop test
+// |scale| max|u| > 1e-16 (the same test on the scaled values, since scaling
+// by |scale| is monotonic) selects the scale or zero.  Zero radial sums give
+// zero values by themselves.
 __device__ __forceinline__ AoVal ao_from_radial(const DevShell& s, const DevAo& ao, double x, double y, double z,
                                                 double2 rr) {
-    AoVal o{0.0, 0.0, 0.0, 0.0};
-    if (rr.x == 0.0 && rr.y == 0.0) return o;
     const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
     double px, dpx, py, dpy, pz, dpz;
-    mono_d(dx, ao.lx, px, dpx);
-    mono_d(dy, ao.ly, py, dpy);
-    mono_d(dz, ao.lz, pz, dpz);
-    const double m = px * py * pz;
-    const double sc = ao.scale;
-    const double v = sc * m * rr.x;
-    const double gx = sc * (dpx * py * pz * rr.x + m * dx * rr.y);
-    const double gy = sc * (px * dpy * pz * rr.x + m * dy * rr.y);
-    const double gz = sc * (px * py * dpz * rr.x + m * dz * rr.y);
-    if (fabs(v) > 1e-16 || fabs(gx) > 1e-16 || fabs(gy) > 1e-16 || fabs(gz) > 1e-16) {
-        o.v = v;
-        o.gx = gx;
-        o.gy = gy;
-        o.gz = gz;
-    }
-    return o;
+    mono_sel(dx, ao.lx, px, dpx);
+    mono_sel(dy, ao.ly, py, dpy);
+    mono_sel(dz, ao.lz, pz, dpz);
+    const double pyz = py * pz, m = px * pyz, mr1 = m * rr.y;
+    const double u = m * rr.x;
+    const double ux = fma(dpx * pyz, rr.x, mr1 * dx);
+    const double uy = fma(px * dpy * pz, rr.x, mr1 * dy);
+    const double uz = fma(px * py * dpz, rr.x, mr1 * dz);
+    const double mx = fmax(fmax(fabs(u), fabs(ux)), fmax(fabs(uy), fabs(uz)));
+    const double sc = fabs(ao.scale) * mx > 1e-16 ? ao.scale : 0.0;
+    return AoVal{sc * u, sc * ux, sc * uy, sc * uz};
 }
 
 // ao_from_radial for a row held in LDS.
@@ -124,10 +129,11 @@ struct AoValF {
 
 __device__ __forceinline__ float exp_neg_f(float x) { return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f); }
 
-__device__ __forceinline__ void mono_df(float x, int l, float& p, float& d) {
-    const float x2 = x * x;
-    p = l == 0 ? 1.0f : (l == 1 ? x : (l == 2 ? x2 : x2 * x));
-    d = l == 0 ? 0.0f : (l == 1 ? 1.0f : (l == 2 ? 2.0f * x : 3.0f * x2));
+// mono_sel in FP32.
+__device__ __forceinline__ void mono_self(float x, int l, float& p, float& d) {
+    const float q = (l & 2) ? ((l & 1) ? x * x : x) : 1.0f;
+    p = l ? x * q : 1.0f;
+    d = static_cast<float>(l) * q;
 }
 
 // Radial sums in FP32 (R0, R1), with the same cutoff and 1e-18 rules.
@@ -152,27 +158,22 @@ __device__ __forceinline__ float2 shell_radial_f(const DevShell& s, const double
 
 __device__ __forceinline__ AoValF ao_from_radial_row_f(const DevShell& s, const RowL& r, double x, double y,
                                                        double z, float2 rr) {
-    AoValF o{0.0f, 0.0f, 0.0f, 0.0f};
-    if (rr.x == 0.0f && rr.y == 0.0f) return o;
+    // straight-line, as ao_from_radial
     const float dx = static_cast<float>(x - s.cx), dy = static_cast<float>(y - s.cy),
                 dz = static_cast<float>(z - s.cz);
     float px, dpx, py, dpy, pz, dpz;
-    mono_df(dx, r.lmn & 3, px, dpx);
-    mono_df(dy, (r.lmn >> 2) & 3, py, dpy);
-    mono_df(dz, (r.lmn >> 4) & 3, pz, dpz);
-    const float m = px * py * pz;
-    const float sc = static_cast<float>(r.scale);
-    const float v = sc * m * rr.x;
-    const float gx = sc * (dpx * py * pz * rr.x + m * dx * rr.y);
-    const float gy = sc * (px * dpy * pz * rr.x + m * dy * rr.y);
-    const float gz = sc * (px * py * dpz * rr.x + m * dz * rr.y);
-    if (fabsf(v) > 1e-16f || fabsf(gx) > 1e-16f || fabsf(gy) > 1e-16f || fabsf(gz) > 1e-16f) {
-        o.v = v;
-        o.gx = gx;
-        o.gy = gy;
-        o.gz = gz;
-    }
-    return o;
+    mono_self(dx, r.lmn & 3, px, dpx);
+    mono_self(dy, (r.lmn >> 2) & 3, py, dpy);
+    mono_self(dz, (r.lmn >> 4) & 3, pz, dpz);
+    const float pyz = py * pz, m = px * pyz, mr1 = m * rr.y;
+    const float u = m * rr.x;
+    const float ux = fmaf(dpx * pyz, rr.x, mr1 * dx);
+    const float uy = fmaf(px * dpy * pz, rr.x, mr1 * dy);
+    const float uz = fmaf(px * py * dpz, rr.x, mr1 * dz);
+    const float mx = fmaxf(fmaxf(fabsf(u), fabsf(ux)), fmaxf(fabsf(uy), fabsf(uz)));
+    const float sc0 = static_cast<float>(r.scale);
+    const float sc = fabsf(sc0) * mx > 1e-16f ? sc0 : 0.0f;
+    return AoValF{sc * u, sc * ux, sc * uy, sc * uz};
 }
 
 // For the nuclear gradient (eval_aos_hess, ao_grid.rs:234-299): radial sums
