@@ -358,6 +358,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
 // Functional stage, one thread per sorted point: from (rho, grad rho) and the
 // weight, wv = (w v_rho_a / 2, w (v_gamma_aa + v_gamma_ab / 2) grad rho) and
 // the workgroup's sums of w f_xc and w rho (vxc.rs:892-920).
+// One instance per functional (F = a.func), so that the functional's code is
+// inlined and its registers are allocated for that functional alone.
+template <int F>
 __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
     __shared__ double se[kThreads], sn[kThreads];
     const int2 wg = a.wg[blockIdx.x];
@@ -371,7 +374,7 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
         if (r.x >= 1e-18) {  // DENSITY_SCREEN, vxc.rs:23
             const double rs = 0.5 * r.x;
             const double gss = 0.25 * (r.y * r.y + r.z * r.z + r.w * r.w);
-            const XcOut o = eval_xc(a.func, rs, rs, gss, gss, gss);
+            const XcOut o = eval_xc(F, rs, rs, gss, gss, gss);
             e = w * o.f;
             const double us = w * (o.gaa + 0.5 * o.gab);
             out = make_double4(0.5 * w * o.va, us * r.y, us * r.z, us * r.w);
@@ -400,6 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
 // gamma_ab, gamma_bb from the spin gradients; u_a = 2 v_gaa grad rho_a +
 // v_gab grad rho_b (and a <-> b); wv_s = (w v_s / 2, w u_s); N sums the
 // unclamped densities.
+template <int F>
 __global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
     __shared__ double se[kThreads], sn[kThreads];
     const int2 wg = a.wg[blockIdx.x];
@@ -415,7 +419,7 @@ __global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
             const double gaa = ra.y * ra.y + ra.z * ra.z + ra.w * ra.w;
             const double gab = ra.y * rb.y + ra.z * rb.z + ra.w * rb.w;
             const double gbb = rb.y * rb.y + rb.z * rb.z + rb.w * rb.w;
-            const XcOut o = eval_xc(a.func, pa, pb, gaa, gab, gbb);
+            const XcOut o = eval_xc(F, pa, pb, gaa, gab, gbb);
             e = w * o.f;
             oa = make_double4(0.5 * w * o.va, w * (2.0 * o.gaa * ra.y + o.gab * rb.y),
                               w * (2.0 * o.gaa * ra.z + o.gab * rb.z), w * (2.0 * o.gaa * ra.w + o.gab * rb.w));
@@ -1174,16 +1178,43 @@ hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t
     return mode == kRhoFp64 ? launch_rho_t<double, false>(a, c, s) : launch_rho_t<float, true>(a, c, s);
 }
 
-hipError_t launch_xc(const XcArgs& a, hipStream_t s) {
+// The kernel instance of functional a.func (an XcId); unknown ids are
+// rejected by the caller before a build.
+template <typename Args, typename Fn>
+static hipError_t launch_by_functional(const Args& a, hipStream_t s, Fn pick) {
     if (a.n_wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xc, dim3(a.n_wg), dim3(kThreads), 0, s, a);
-    return hipGetLastError();
+    const void* k = pick(a.func);
+    if (!k) return hipErrorInvalidValue;
+    void* args[] = {const_cast<Args*>(&a)};
+    return hipLaunchKernel(k, dim3(a.n_wg), dim3(kThreads), args, 0, s);
+}
+
+hipError_t launch_xc(const XcArgs& a, hipStream_t s) {
+    return launch_by_functional(a, s, [](int f) -> const void* {
+        switch (f) {
+        case XC_SVWN5: return reinterpret_cast<const void*>(&k_xc<XC_SVWN5>);
+        case XC_BLYP: return reinterpret_cast<const void*>(&k_xc<XC_BLYP>);
+        case XC_PBE: return reinterpret_cast<const void*>(&k_xc<XC_PBE>);
+        case XC_B3LYP: return reinterpret_cast<const void*>(&k_xc<XC_B3LYP>);
+        case XC_B3LYP5: return reinterpret_cast<const void*>(&k_xc<XC_B3LYP5>);
+        case XC_PBE0: return reinterpret_cast<const void*>(&k_xc<XC_PBE0>);
+        default: return nullptr;
+        }
+    });
 }
 
 hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s) {
-    if (a.n_wg <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_xc_uks, dim3(a.n_wg), dim3(kThreads), 0, s, a);
-    return hipGetLastError();
+    return launch_by_functional(a, s, [](int f) -> const void* {
+        switch (f) {
+        case XC_SVWN5: return reinterpret_cast<const void*>(&k_xc_uks<XC_SVWN5>);
+        case XC_BLYP: return reinterpret_cast<const void*>(&k_xc_uks<XC_BLYP>);
+        case XC_PBE: return reinterpret_cast<const void*>(&k_xc_uks<XC_PBE>);
+        case XC_B3LYP: return reinterpret_cast<const void*>(&k_xc_uks<XC_B3LYP>);
+        case XC_B3LYP5: return reinterpret_cast<const void*>(&k_xc_uks<XC_B3LYP5>);
+        case XC_PBE0: return reinterpret_cast<const void*>(&k_xc_uks<XC_PBE0>);
+        default: return nullptr;
+        }
+    });
 }
 
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {

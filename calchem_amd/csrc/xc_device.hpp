@@ -337,7 +337,7 @@ __device__ __forceinline__ XcOut pbe_correlation(double ra, double rb, double ga
 }
 
 // eval_xc_spin (xc.rs:159-223).
-__device__ inline XcOut eval_xc(int func, double ra_in, double rb_in, double gaa_in, double gab, double gbb_in) {
+__device__ __forceinline__ XcOut eval_xc(int func, double ra_in, double rb_in, double gaa_in, double gab, double gbb_in) {
     const double ra = fmax(ra_in, 0.0), rb = fmax(rb_in, 0.0);
     if (ra + rb < kRhoCut) return xc_zero();
     const double gaa = fmax(gaa_in, 0.0), gbb = fmax(gbb_in, 0.0);
