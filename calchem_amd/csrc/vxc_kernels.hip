@@ -355,6 +355,33 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     }
 }
 
+// Sums of e and n over the workgroup (kThreads threads) into *oe and *on, in
+// a fixed order: within each wave by xor shuffles, then the waves in order.
+__device__ __forceinline__ void wg_sum2(double e, double n, double* oe, double* on) {
+    __shared__ double sw[2][kThreads / 64];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        e += __shfl_xor(e, o);
+        n += __shfl_xor(n, o);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        sw[0][wave] = e;
+        sw[1][wave] = n;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double te = 0.0, tn = 0.0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; ++w) {
+            te += sw[0][w];
+            tn += sw[1][w];
+        }
+        *oe = te;
+        *on = tn;
+    }
+}
+
 // Functional stage, one thread per sorted point: from (rho, grad rho) and the
 // weight, wv = (w v_rho_a / 2, w (v_gamma_aa + v_gamma_ab / 2) grad rho) and
 // the workgroup's sums of w f_xc and w rho (vxc.rs:892-920).
@@ -362,7 +389,6 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
 // inlined and its registers are allocated for that functional alone.
 template <int F>
 __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
-    __shared__ double se[kThreads], sn[kThreads];
     const int2 wg = a.wg[blockIdx.x];
     const int g = wg.x + threadIdx.x;
     double e = 0.0, ne = 0.0;
@@ -381,20 +407,7 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
         }
         a.wv[g] = out;
     }
-    se[threadIdx.x] = e;
-    sn[threadIdx.x] = ne;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            se[threadIdx.x] += se[threadIdx.x + s];
-            sn[threadIdx.x] += sn[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.e_part[blockIdx.x] = se[0];
-        a.n_part[blockIdx.x] = sn[0];
-    }
+    wg_sum2(e, ne, a.e_part + blockIdx.x, a.n_part + blockIdx.x);
 }
 
 // Unrestricted functional stage, one thread per sorted point
@@ -405,7 +418,6 @@ __global__ __launch_bounds__(kThreads) void k_xc(XcArgs a) {
 // unclamped densities.
 template <int F>
 __global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
-    __shared__ double se[kThreads], sn[kThreads];
     const int2 wg = a.wg[blockIdx.x];
     const int g = wg.x + threadIdx.x;
     double e = 0.0, ne = 0.0;
@@ -429,20 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_xc_uks(XcUksArgs a) {
         a.wv_a[g] = oa;
         a.wv_b[g] = ob;
     }
-    se[threadIdx.x] = e;
-    sn[threadIdx.x] = ne;
-    __syncthreads();
-    for (int s = kThreads / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) {
-            se[threadIdx.x] += se[threadIdx.x + s];
-            sn[threadIdx.x] += sn[threadIdx.x + s];
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        a.e_part[blockIdx.x] = se[0];
-        a.n_part[blockIdx.x] = sn[0];
-    }
+    wg_sum2(e, ne, a.e_part + blockIdx.x, a.n_part + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
