@@ -187,14 +187,19 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         }
         __syncthreads();
 
-        // (phi, grad phi) of row Mma::row(kq, v) of row tile it at point col.
+        // (phi, grad phi) of row Mma::row(kq, v) of row tile it at point col,
+        // without a per-lane branch: padding rows (shell -1, scale 0) read
+        // shell 0 and come out zero, and so do points past the tile's end
+        // (zero radial sums).  A block without rows has no shell 0 and skips
+        // this as a whole.
         auto eval = [&](int it, int v) {
-            const int r = (it << 4) + M::row(kq, v);
-            const RowL row = sRow[r];
             AoVal f{0.0, 0.0, 0.0, 0.0};
-            if (row.sh >= 0 && col < np) {
-                const double4 x = sPt[col];
-                f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + col]);
+            if (nsh > 0) {
+                const int r = (it << 4) + M::row(kq, v);
+                const RowL row = sRow[r];
+                const int sh = max(row.sh, 0);
+                const double4 xp = sPt[col];
+                f = ao_from_radial_row(sSh[sh], row, xp.x, xp.y, xp.z, sRad[sh * PTS + col]);
             }
             return f;
         };
@@ -696,15 +701,14 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
         const double4* wv = sWv + (st % kVSlots) * kVPts;
         for (int idx = tid; idx < mp * kVPts; idx += NT) {
             const int r = idx / kVPts, p = idx % kVPts;
+            // padding rows (shell -1, scale 0) read shell 0 and come out zero
             const RowL row = sRow[r];
-            T v = 0, z = 0;
-            if (row.sh >= 0) {
-                const double4 x = pt[p];
-                const double4 w = wv[p];
-                const AoVal f = ao_from_radial_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * kVPts + p]);
-                v = f.v;
-                z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
-            }
+            const int sh = max(row.sh, 0);
+            const double4 x = pt[p];
+            const double4 w = wv[p];
+            const AoVal f = ao_from_radial_row(sSh[sh], row, x.x, x.y, x.z, sRad[sh * kVPts + p]);
+            const T v = f.v;
+            const T z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
             sP[r * LD + p] = v;
             sZ[r * LD + p] = st * kVPts + p < npts ? z : T(0);
         }
@@ -861,16 +865,15 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         T* Z = P + S;
         for (int idx = tid; idx < mp * kVPts; idx += NT) {
             const int r = idx / kVPts, p = idx % kVPts;
+            // padding rows (shell -1, scale 0) read shell 0 and come out zero
             const RowL row = sRow[r];
-            T v = 0, z = 0;
-            if (row.sh >= 0) {
-                const double4 x = pt[p];
-                const double4 w = wv[p];
-                const AoValF f = ao_from_radial_row_f(sSh[row.sh], row, x.x, x.y, x.z, rad[row.sh * kVPts + p]);
-                v = f.v;
-                z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx + static_cast<float>(w.z) * f.gy +
-                    static_cast<float>(w.w) * f.gz;
-            }
+            const int sh = max(row.sh, 0);
+            const double4 x = pt[p];
+            const double4 w = wv[p];
+            const AoValF f = ao_from_radial_row_f(sSh[sh], row, x.x, x.y, x.z, rad[sh * kVPts + p]);
+            const T v = f.v;
+            const T z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx +
+                        static_cast<float>(w.z) * f.gy + static_cast<float>(w.w) * f.gz;
             P[r * LD + p] = v;
             Z[r * LD + p] = st * kVPts + p < npts ? z : T(0);
         }
