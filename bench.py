@@ -67,6 +67,24 @@ KERNEL_SOURCES = ("calchem_amd/csrc/vxc_kernels.hip", "calchem_amd/csrc/api.hip"
                   "calchem_amd/csrc/common.hpp", "calchem_amd/csrc/kernels.hpp")
 
 
+def measured_mfma_peak(prec: str, n_wg: int = 2048, iters: int = 2000) -> float:
+    """Dense MFMA ceiling measured on this device by calchem_amd/libcalchem_probe.so
+    (four independent 16x16x4 MFMA chains per wave, operands in registers)."""
+    import ctypes
+    path = os.path.join(ROOT, "calchem_amd", "libcalchem_probe.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: run __graft_entry__.build() first")
+    lib = ctypes.CDLL(path)
+    lib.calchem_probe_mfma_tflops.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_double)]
+    lib.calchem_probe_mfma_tflops.restype = ctypes.c_int
+    out = ctypes.c_double(0.0)
+    rc = lib.calchem_probe_mfma_tflops(0 if prec == "fp64" else 1, n_wg, iters, ctypes.byref(out))
+    if rc != 0:
+        raise RuntimeError(f"calchem_probe_mfma_tflops failed with {rc}")
+    return out.value
+
+
 def sources_digest() -> str:
     """sha256 over the kernel and plan sources: a traffic summary applies only to the code it was measured on."""
     h = hashlib.sha256()
@@ -380,6 +398,8 @@ def main() -> int:
     kname = "k_rho" if dom == "rho" else ("k_vxc_group" if st["n_v_pairs"] == 0 else "k_vxc_group+k_vxc")
     achieved = flops[dom] / (kms[dom] * 1e-3) / 1e12
     peak = PEAK_TFLOPS[best]
+    measured_peak = measured_mfma_peak(best)
+    log(f"measured {best} MFMA ceiling: {measured_peak:.1f} TFLOP/s (datasheet {peak})")
     traffic = load_traffic(args.workload, best, kname)
     build_tflops = (st["flops_rho"] + st["flops_v"]) / (rb["ms"] * 1e-3) / 1e12
 
@@ -435,6 +455,9 @@ def main() -> int:
         "kernel_ms": kms,
         "roofline": {"bound": "mfma", "kernel": kname, "launches": "all launches of one build",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "measured_peak": measured_peak, "frac_of_measured_peak": achieved / measured_peak,
+                     "measured_peak_source": "calchem_amd/csrc/probe_mfma.hip: 2048 workgroups x 4 waves x "
+                                             "2000 rounds of 4 independent 16x16x4 MFMA chains, one timed launch",
                      "algorithmic_tflop_per_build": flops[dom] / 1e12, "kernel_ms_per_build": kms[dom],
                      "traffic": traffic["hbm_bytes_per_build"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None},

@@ -1,4 +1,5 @@
-"""Build the gfx950 HIP library calchem_amd/libcalchem_hip.so with hipcc.
+"""Build the gfx950 HIP library calchem_amd/libcalchem_hip.so with hipcc, and
+the MFMA throughput probe calchem_amd/libcalchem_probe.so used by bench.py.
 
 Every call compiles from source (no incremental reuse), so the library always
 matches the tree.  Also used by __graft_entry__.build().
@@ -14,6 +15,8 @@ import tempfile
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcalchem_hip.so")
+PROBE_LIB = os.path.join(HERE, "libcalchem_probe.so")
+PROBE_SOURCE = "probe_mfma.hip"
 SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip"]
 ARCH = os.environ.get("CALCHEM_HIP_ARCH", "gfx950")
 
@@ -56,6 +59,14 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("hipcc link of libcalchem_hip.so failed")
         os.replace(out_tmp, LIB)
+        # the measurement probe: a library of its own, outside the Vxc interface
+        probe_tmp = PROBE_LIB + ".tmp"
+        cmd = [cc, *HIPCC_FLAGS, "-shared", "-o", probe_tmp, os.path.join(CSRC, PROBE_SOURCE)]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc build of libcalchem_probe.so failed")
+        os.replace(probe_tmp, PROBE_LIB)
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
     return LIB

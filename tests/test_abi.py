@@ -42,6 +42,22 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name)
 
 
+def test_probe_library_exports_its_entry_point():
+    # the MFMA throughput probe that bench.py reports beside the datasheet peak
+    path = os.path.join(ROOT, "calchem_amd", "libcalchem_probe.so")
+    if not os.path.exists(path):
+        pytest.skip("libcalchem_probe.so not built (run __graft_entry__.build())")
+    out = subprocess.run(["nm", "-D", "--defined-only", path], capture_output=True, text=True, check=True).stdout
+    assert re.search(r"\sT\scalchem_probe_mfma_tflops\b", out)
+    lib = C.CDLL(path)
+    lib.calchem_probe_mfma_tflops.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    out_v = C.c_double(0.0)
+    # argument checks come before any device work
+    assert lib.calchem_probe_mfma_tflops(2, 1, 1, C.byref(out_v)) == -1
+    assert lib.calchem_probe_mfma_tflops(0, 0, 1, C.byref(out_v)) == -1
+    assert lib.calchem_probe_mfma_tflops(0, 1, 0, C.byref(out_v)) == -1
+
+
 def _c_layout():
     src = r"""
 #include <stddef.h>
