@@ -123,9 +123,6 @@ __device__ __forceinline__ AoVal ao_from_radial_row(const DevShell& s, const Row
 // centre are formed in FP64 and rounded once, so r^2 carries FP32 relative
 // error only; e^{-x} then has absolute error below ~1e-7 x e^{-x} <= 4e-8.
 // e^{-x} uses the hardware base-2 exponential (v_exp_f32).
-struct AoValF {
-    float v, gx, gy, gz;
-};
 
 __device__ __forceinline__ float exp_neg_f(float x) { return __builtin_amdgcn_exp2f(-x * 1.4426950408889634f); }
 
@@ -136,9 +133,19 @@ __device__ __forceinline__ void mono_self(float x, int l, float& p, float& d) {
     d = static_cast<float>(l) * q;
 }
 
-// Radial sums in FP32 (R0, R1), with the same cutoff and 1e-18 rules.
-__device__ __forceinline__ float2 shell_radial_f(const DevShell& s, const double2* __restrict__ prims, double x,
-                                                 double y, double z) {
+// For the mixed build's V stage, the AO values only enter through
+// P = phi and Z = w0 phi + w1 dphi/dx + w2 dphi/dy + w3 dphi/dz, with
+// (w0, w1, w2, w3) = wv of the point.  With phi = s m R0 and
+// grad phi = s (grad(m) R0 + m R1 d) (m = x^lx y^ly z^lz, d = r - centre):
+//   Z = s (m A + R0 B),  A = w0 R0 + R1 (w1 dx + w2 dy + w3 dz),
+//   B = w1 dm/dx + w2 dm/dy + w3 dm/dz,
+// so A is formed once per (shell, point) beside the radial sums, and only m
+// and B per row.  No per-AO 1e-16 drop test here: what it removes is below
+// FP32 resolution of these sums.
+// Radial pair (R0, A) of one shell at one point, FP32, with the cutoff and
+// 1e-18 rules of shell_radial.
+__device__ __forceinline__ float2 shell_radial_za_f(const DevShell& s, const double2* __restrict__ prims, double x,
+                                                    double y, double z, const float4& w) {
     const float dx = static_cast<float>(x - s.cx), dy = static_cast<float>(y - s.cy),
                 dz = static_cast<float>(z - s.cz);
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -153,27 +160,22 @@ __device__ __forceinline__ float2 shell_radial_f(const DevShell& s, const double
         }
         if (fabsf(r0) < 1e-18f && fabsf(r1) < 1e-18f) r0 = r1 = 0.0f;
     }
-    return make_float2(r0, r1);
+    return make_float2(r0, fmaf(w.x, r0, r1 * fmaf(w.y, dx, fmaf(w.z, dy, w.w * dz))));
 }
 
-__device__ __forceinline__ AoValF ao_from_radial_row_f(const DevShell& s, const RowL& r, double x, double y,
-                                                       double z, float2 rr) {
-    // straight-line, as ao_from_radial
+// (P, Z) of one row at one point from its shell's (R0, A).
+__device__ __forceinline__ float2 pz_from_radial_row_f(const DevShell& s, const RowL& r, double x, double y, double z,
+                                                       float2 ra, const float4& w) {
     const float dx = static_cast<float>(x - s.cx), dy = static_cast<float>(y - s.cy),
                 dz = static_cast<float>(z - s.cz);
     float px, dpx, py, dpy, pz, dpz;
     mono_self(dx, r.lmn & 3, px, dpx);
     mono_self(dy, (r.lmn >> 2) & 3, py, dpy);
     mono_self(dz, (r.lmn >> 4) & 3, pz, dpz);
-    const float pyz = py * pz, m = px * pyz, mr1 = m * rr.y;
-    const float u = m * rr.x;
-    const float ux = fmaf(dpx * pyz, rr.x, mr1 * dx);
-    const float uy = fmaf(px * dpy * pz, rr.x, mr1 * dy);
-    const float uz = fmaf(px * py * dpz, rr.x, mr1 * dz);
-    const float mx = fmaxf(fmaxf(fabsf(u), fabsf(ux)), fmaxf(fabsf(uy), fabsf(uz)));
-    const float sc0 = static_cast<float>(r.scale);
-    const float sc = fabsf(sc0) * mx > 1e-16f ? sc0 : 0.0f;
-    return AoValF{sc * u, sc * ux, sc * uy, sc * uz};
+    const float pyz = py * pz, m = px * pyz;
+    const float b = fmaf(w.y * dpx, pyz, px * fmaf(w.z * dpy, pz, w.w * py * dpz));
+    const float sc = static_cast<float>(r.scale);
+    return make_float2(sc * (m * ra.x), sc * fmaf(m, ra.y, ra.x * b));
 }
 
 // For the nuclear gradient (eval_aos_hess, ao_grid.rs:234-299): radial sums

@@ -848,13 +848,19 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
             (tid < kVPts ? sPt : sWv)[slot * kVPts + j] = v;
         }
     };
+    // (R0, A) per (shell, point) of step st, A = w0 R0 + R1 (w . d): the
+    // points and wv of step st arrived two intervals earlier
     auto radial = [&](int st) {
         const double4* pt = sPt + (st % kVSlots) * kVPts;
+        const double4* wv = sWv + (st % kVSlots) * kVPts;
         float2* rad = sRad + (st & 1) * nsh * kVPts;
         for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
-            const double4 x = pt[p];
-            rad[idx] = st * kVPts + p < npts ? shell_radial_f(sSh[sh], sPrim, x.x, x.y, x.z) : make_float2(0.0f, 0.0f);
+            const double4 x = pt[p], wd = wv[p];
+            const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y), static_cast<float>(wd.z),
+                                         static_cast<float>(wd.w));
+            rad[idx] = st * kVPts + p < npts ? shell_radial_za_f(sSh[sh], sPrim, x.x, x.y, x.z, w)
+                                             : make_float2(0.0f, 0.0f);
         }
     };
     auto evaluate = [&](int st) {
@@ -865,17 +871,16 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         T* Z = P + S;
         for (int idx = tid; idx < mp * kVPts; idx += NT) {
             const int r = idx / kVPts, p = idx % kVPts;
-            // padding rows (shell -1, scale 0) read shell 0 and come out zero
+            // padding rows (shell -1, scale 0) read shell 0 and come out zero,
+            // and so do points past the group's end (zero (R0, A))
             const RowL row = sRow[r];
             const int sh = max(row.sh, 0);
-            const double4 x = pt[p];
-            const double4 w = wv[p];
-            const AoValF f = ao_from_radial_row_f(sSh[sh], row, x.x, x.y, x.z, rad[sh * kVPts + p]);
-            const T v = f.v;
-            const T z = static_cast<float>(w.x) * f.v + static_cast<float>(w.y) * f.gx +
-                        static_cast<float>(w.z) * f.gy + static_cast<float>(w.w) * f.gz;
-            P[r * LD + p] = v;
-            Z[r * LD + p] = st * kVPts + p < npts ? z : T(0);
+            const double4 x = pt[p], wd = wv[p];
+            const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y), static_cast<float>(wd.z),
+                                         static_cast<float>(wd.w));
+            const float2 pz = pz_from_radial_row_f(sSh[sh], row, x.x, x.y, x.z, rad[sh * kVPts + p], w);
+            P[r * LD + p] = pz.x;
+            Z[r * LD + p] = pz.y;
         }
     };
     using M = Mma<T>;
