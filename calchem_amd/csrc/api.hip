@@ -91,10 +91,10 @@ struct Plan {
         group_rows,
         group_meta, red_part, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
-        V, Vb_host, rho, rho_s, psub64, psub32, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom, atom_xyz, inv_r,
-        asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
+        V, Vb_host, rho, rho_s, psub64, psub32, psub64b, psub32b, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom,
+        atom_xyz, inv_r, asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
     int n_atoms = 0;  // atoms set by calchem_hip_vxc_set_atoms (0 = not set)
-    bool uks_ready = false;  // rho_sb / wv_b allocated (first unrestricted build)
+    bool uks_ready = false;  // rho_sb / wv_b / psub64b / psub32b allocated (first unrestricted build)
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -122,7 +122,7 @@ struct Plan {
                              &block_sh, &block_meta, &group_meta, &red_part, &rho_tiles, &grad_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
-                             &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
+                             &psub64b, &psub32b, &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
                              &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
@@ -793,10 +793,19 @@ int fan_out(Plan& P, hipStream_t s, int n, F fn) {
     return kOk;
 }
 
-int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark) {
+int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark,
+            double4* sorted_b = nullptr, const double* dPb = nullptr) {
+    const bool two = dPb != nullptr;
     GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP, P.psub64.as<double>(),
                   mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
     if (launch_gather_p(ga, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
+    if (two) {
+        GatherArgs gb = ga;
+        gb.P = dPb;
+        gb.out64 = P.psub64b.as<double>();
+        gb.out32 = mode == kRhoMixed ? P.psub32b.as<float>() : nullptr;
+        if (launch_gather_p(gb, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
+    }
     if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
@@ -808,7 +817,21 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
                    P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
                    P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(),
                    sorted ? 0 : 1};
-        return launch_rho(ra, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
+        if (!two) return launch_rho(ra, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
+        // both spins from one AO evaluation when the class's LDS allows,
+        // else one pass per spin
+        if (rho_lds_bytes(mode, P.classes[c], 2) <= P.lds_limit) {
+            ra.P64b = P.psub64b.as<double>();
+            ra.PSb = mode == kRhoMixed ? static_cast<const void*>(P.psub32b.p) : P.psub64b.p;
+            ra.rho_out_b = sorted_b;
+            return launch_rho(ra, mode, P.classes[c], st, 2) == hipSuccess ? kOk : kLaunchFail;
+        }
+        if (launch_rho(ra, mode, P.classes[c], st) != hipSuccess) return kLaunchFail;
+        RhoArgs rb = ra;
+        rb.P64 = P.psub64b.as<double>();
+        rb.PS = mode == kRhoMixed ? static_cast<const void*>(P.psub32b.p) : P.psub64b.p;
+        rb.rho_out = sorted_b;
+        return launch_rho(rb, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
     });
 }
 
@@ -873,7 +896,10 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
     if (!P.has_weights) return kBadArgs;
     if (!P.uks_ready) {
         const size_t b = sizeof(double4) * (size_t)std::max(P.n_pts, 1);
-        if (!P.rho_sb.alloc(b) || !P.wv_b.alloc(b)) return kAllocFail;
+        if (!P.rho_sb.alloc(b) || !P.wv_b.alloc(b) ||
+            !P.psub64b.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)) ||
+            !P.psub32b.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail)))
+            return kAllocFail;
         // Points of blocks without significant AOs are never written.
         if (hipMemset(P.rho_sb.p, 0, b) != hipSuccess) return kLaunchFail;
         P.uks_ready = true;
@@ -881,8 +907,7 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
     const bool f32 = prec == CALCHEM_PREC_FP32;
     const int mode = f32 ? kRhoMixed : kRhoFp64;
     mark(P, s);
-    int rc = run_rho(P, mode, P.rho_s.as<double4>(), dPa, s, true);
-    if (rc == kOk) rc = run_rho(P, mode, P.rho_sb.as<double4>(), dPb, s, false);
+    int rc = run_rho(P, mode, P.rho_s.as<double4>(), dPa, s, true, P.rho_sb.as<double4>(), dPb);
     if (rc != kOk) return rc;
     mark(P, s);
     XcUksArgs xa{P.rho_s.as<double4>(), P.rho_sb.as<double4>(), P.pts.as<double4>(), P.xc_wg.as<int2>(), P.n_xc_wg, func,

@@ -20,6 +20,10 @@ struct RhoArgs {
     const int* perm;        // sorted point -> caller point
     int rho_perm;
     int seg[kXcds + 1];     // this launch's XCD segments (RhoClass::seg)
+    // two-spin launches only: the beta density blocks and beta output
+    const double* P64b;
+    const void* PSb;
+    double4* rho_out_b;
 };
 
 // Functional-stage workgroups never straddle molecules: workgroup w covers
@@ -184,8 +188,8 @@ struct SigArgs {
 // Rho kernel variants: FP64 throughout, or mixed (leading rows FP64, the
 // rest FP32).
 enum RhoMode { kRhoFp64 = 0, kRhoMixed = 1 };
-size_t rho_lds_bytes(int mode, const RhoClass& c);
-hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s);
+size_t rho_lds_bytes(int mode, const RhoClass& c, int ns = 1);
+hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s, int ns = 1);
 hipError_t launch_xc(const XcArgs& a, hipStream_t s);
 hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);

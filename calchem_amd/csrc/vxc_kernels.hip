@@ -77,8 +77,8 @@ struct Vec4<float> {
 // multiple of 16 bytes): points, per-wave sums, radial table [shell][point],
 // rows, shells, primitives (the block's packed records, copied as one run),
 // FP64 leading-row values, row values.
-__host__ __device__ constexpr size_t rho_lds_fixed() {
-    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * kRhoPts;
+__host__ __device__ constexpr size_t rho_lds_fixed(int ns) {
+    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * ns * kRhoPts;
 }
 
 // Workgroups are dispatched round robin over the XCDs (workgroup i to XCD
@@ -102,7 +102,11 @@ __device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >
 // chunks ahead), rho += phi T and grad rho += 2 grad phi T, summed over rows
 // with cross-lane shuffles and then over the waves.
 
-template <typename TS, bool HYB>
+//
+// NS = 2 (unrestricted builds): the same AO values serve both spin densities;
+// each row tile is contracted with the alpha and then the beta density block
+// and both sets of sums are formed (rho_out gets alpha, rho_out_b beta).
+template <typename TS, bool HYB, int NS>
 __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = 2;
@@ -115,8 +119,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     unsigned char* q = smem;
     double4* sPtAll = reinterpret_cast<double4*>(q);  // [kRhoSub * PTS]
     q += sizeof(double4) * kRhoSub * kRhoPts;
-    double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 sums][PTS]
-    q += sizeof(double) * 16 * PTS;
+    double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 NS sums][PTS]
+    q += sizeof(double) * 16 * NS * PTS;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
     q += sizeof(double2) * nsh * PTS;
     RowL* sRow = reinterpret_cast<RowL*>(q);
@@ -134,6 +138,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     const int col = lane & 15, kq = lane >> 4;
     using M = Mma<TS>;
     const TS* __restrict__ PS = static_cast<const TS*>(a.PS);
+    const TS* __restrict__ PSb = static_cast<const TS*>(a.PSb);
     using V4 = typename Vec4<TS>::type;
     // The first row tile's first density chunks are loaded now, so that their
     // latency passes during the staging and AO evaluation below (the loads
@@ -229,13 +234,15 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         __syncthreads();
 
         // T for row tile it: this lane ends up with T[row(kq, v)][col].
-        auto contract = [&](int it, typename M::acc& acc, v4d& acc64, bool first) {
+        auto contract = [&](int it, typename M::acc& acc, v4d& acc64, bool first, int spin) {
+            const TS* __restrict__ PSs = (NS == 2 && spin == 1) ? PSb : PS;
+            const double* __restrict__ P64s = (NS == 2 && spin == 1) ? a.P64b : a.P64;
             const int i0 = it << 4;
             acc = M::zero();
             // The first two density chunks of the rows in the precision TS are
             // requested before the FP64 leading part, so that they arrive
             // while it runs.
-            const TS* __restrict__ Ap = PS + blk.psub + static_cast<long long>(i0 + col) * mp + m64 + 4 * kq;
+            const TS* __restrict__ Ap = PSs + blk.psub + static_cast<long long>(i0 + col) * mp + m64 + 4 * kq;
             auto load = [&](int c) { return *reinterpret_cast<const V4*>(Ap + 16 * c); };
             V4 r0 = first ? pre0 : load(0);
             V4 r1 = first ? pre1 : load(1);
@@ -245,7 +252,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
                 // Leading rows (large |phi|) in FP64 with the FP64 density
                 // block, four k-steps per round with their loads issued together.
                 const double* __restrict__ Arow =
-                    a.P64 + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
+                    P64s + blk.psub + static_cast<long long>(i0 + f64_row_as_f32(col)) * mp;
                 int k = 0;
                 for (; k < m64; k += 16) {
                     double av[4];
@@ -296,60 +303,78 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             if (c < nch) chunk(r0, c);
             if (c + 1 < nch) chunk(r1, c + 1);
         };
-        double sr = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
-        auto accumulate = [&](const typename M::acc& acc, const v4d& acc64, int v, const AoVal& f) {
+        // sums[s] = (rho, grad rho / 2) of spin s at this lane's point
+        double sums[NS][4];
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sums[sp][k] = 0.0;
+        auto accumulate = [&](int sp, const typename M::acc& acc, const v4d& acc64, int v, const AoVal& f) {
             double t = static_cast<double>(acc[v]);
             if constexpr (HYB) t += acc64[v];
-            sr += f.v * t;
-            sx += f.gx * t;
-            sy += f.gy * t;
-            sz += f.gz * t;
+            sums[sp][0] += f.v * t;
+            sums[sp][1] += f.gx * t;
+            sums[sp][2] += f.gy * t;
+            sums[sp][3] += f.gz * t;
         };
 #pragma unroll
         for (int j = 0; j < kRhoCached; ++j) {
             const int it = wave + 4 * j;
             if (it < T16) {
-                typename M::acc acc;
-                v4d acc64;
-                contract(it, acc, acc64, j == 0);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, cache[j][v]);
+                for (int sp = 0; sp < NS; ++sp) {
+                    typename M::acc acc;
+                    v4d acc64;
+                    contract(it, acc, acc64, j == 0 && sp == 0, sp);
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) accumulate(sp, acc, acc64, v, cache[j][v]);
+                }
             }
         }
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
-            typename M::acc acc;
-            v4d acc64;
-            contract(it, acc, acc64, false);
+            typename M::acc acc[NS];
+            v4d acc64[NS];
 #pragma unroll
-            for (int v = 0; v < 4; ++v) accumulate(acc, acc64, v, eval(it, v));
+            for (int sp = 0; sp < NS; ++sp) contract(it, acc[sp], acc64[sp], false, sp);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const AoVal f = eval(it, v);
+#pragma unroll
+                for (int sp = 0; sp < NS; ++sp) accumulate(sp, acc[sp], acc64[sp], v, f);
+            }
         }
         // Sum over the four row groups of the wave, then over the waves.
-        sr += __shfl_xor(sr, 16);
-        sr += __shfl_xor(sr, 32);
-        sx += __shfl_xor(sx, 16);
-        sx += __shfl_xor(sx, 32);
-        sy += __shfl_xor(sy, 16);
-        sy += __shfl_xor(sy, 32);
-        sz += __shfl_xor(sz, 16);
-        sz += __shfl_xor(sz, 32);
+        // (all shuffles first, then the stores under one branch)
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sums[sp][k] += __shfl_xor(sums[sp][k], 16);
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sums[sp][k] += __shfl_xor(sums[sp][k], 32);
         if (kq == 0) {
-            sRed[(wave * 4 + 0) * PTS + col] = sr;
-            sRed[(wave * 4 + 1) * PTS + col] = sx;
-            sRed[(wave * 4 + 2) * PTS + col] = sy;
-            sRed[(wave * 4 + 3) * PTS + col] = sz;
+#pragma unroll
+            for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sRed[(wave * 4 * NS + 4 * sp + k) * PTS + col] = sums[sp][k];
         }
         __syncthreads();
+        double4* const outs[2] = {a.rho_out, a.rho_out_b};
         if (tid < np) {
-            double rho = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
-#pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                rho += sRed[(w * 4 + 0) * PTS + tid];
-                gx += sRed[(w * 4 + 1) * PTS + tid];
-                gy += sRed[(w * 4 + 2) * PTS + tid];
-                gz += sRed[(w * 4 + 3) * PTS + tid];
-            }
             const int g = blk.pt_begin + tsub + tid;
-            a.rho_out[a.rho_perm ? a.perm[g] : g] = make_double4(rho, 2.0 * gx, 2.0 * gy, 2.0 * gz);
+            const int go = a.rho_perm ? a.perm[g] : g;
+#pragma unroll
+            for (int sp = 0; sp < NS; ++sp) {
+                double r[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[k] = sRed[(4 * sp + k) * PTS + tid];
+#pragma unroll
+                for (int w = 1; w < 4; ++w)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) r[k] += sRed[(w * 4 * NS + 4 * sp + k) * PTS + tid];
+                outs[sp][go] = make_double4(r[0], 2.0 * r[1], 2.0 * r[2], 2.0 * r[3]);
+            }
         }
     };
     static_assert(kRhoSub == 2, "two sub-tiles per workgroup");
@@ -1146,16 +1171,16 @@ __global__ void k_significance(SigArgs a) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-template <typename TS, bool HYB>
+template <typename TS, bool HYB, int NS>
 static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
-    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, c);
+    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, c, NS);
     static int attr_set = 0;
     if (!attr_set) {
         int dev = 0, lim = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, HYB>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, HYB, NS>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
         attr_set = 1;
@@ -1169,20 +1194,22 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
         longest = std::max(longest, c.seg[x + 1] - c.seg[x]);
     }
     b.seg[kXcds] = c.seg[kXcds];
-    hipLaunchKernelGGL((k_rho<TS, HYB>), dim3(kXcds * longest), dim3(kThreads), lds, s, b);
+    hipLaunchKernelGGL((k_rho<TS, HYB, NS>), dim3(kXcds * longest), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
-size_t rho_lds_bytes(int mode, const RhoClass& c) {
+size_t rho_lds_bytes(int mode, const RhoClass& c, int ns) {
     const size_t ts = mode == kRhoMixed ? sizeof(float) : sizeof(double);
-    return rho_lds_fixed() + sizeof(double2) * (size_t)c.max_nsh * kRhoPts + sizeof(DevShell) * (size_t)c.max_nsh +
+    return rho_lds_fixed(ns) + sizeof(double2) * (size_t)c.max_nsh * kRhoPts + sizeof(DevShell) * (size_t)c.max_nsh +
            sizeof(double2) * (size_t)c.max_prim + sizeof(RowL) * (size_t)c.max_mpad +
            (mode == kRhoMixed ? sizeof(double) * (size_t)c.max_m64 * kRhoPts : 0) +
            ts * (size_t)c.max_mpad * kRhoPts;
 }
 
-hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s) {
-    return mode == kRhoFp64 ? launch_rho_t<double, false>(a, c, s) : launch_rho_t<float, true>(a, c, s);
+hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s, int ns) {
+    if (ns == 2)
+        return mode == kRhoFp64 ? launch_rho_t<double, false, 2>(a, c, s) : launch_rho_t<float, true, 2>(a, c, s);
+    return mode == kRhoFp64 ? launch_rho_t<double, false, 1>(a, c, s) : launch_rho_t<float, true, 1>(a, c, s);
 }
 
 // The kernel instance of functional a.func (an XcId); unknown ids are
