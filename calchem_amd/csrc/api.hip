@@ -897,7 +897,7 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
     if (!P.uks_ready) {
         const size_t b = sizeof(double4) * (size_t)std::max(P.n_pts, 1);
         if (!P.rho_sb.alloc(b) || !P.wv_b.alloc(b) ||
-            !P.psub64b.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)) ||
+            (!P.psub64b.p && !P.psub64b.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail))) ||
             !P.psub32b.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail)))
             return kAllocFail;
         // Points of blocks without significant AOs are never written.
@@ -1321,9 +1321,10 @@ int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double* Pa, const
         return kLaunchFail;
     const double* dPa = P.P64.as<double>();
     const double* dPb = uks ? P.Pb_host.as<double>() : nullptr;
-    // Densities at the points, then the functional.
-    int rc = run_rho(P, kRhoFp64, P.rho_s.as<double4>(), dPa, s, false);
-    if (rc == kOk && uks) rc = run_rho(P, kRhoFp64, P.rho_sb.as<double4>(), dPb, s, false);
+    // Densities at the points (both spins from one AO evaluation), then the functional.
+    if (uks && !P.psub64b.p && !P.psub64b.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail)))
+        return kAllocFail;
+    int rc = run_rho(P, kRhoFp64, P.rho_s.as<double4>(), dPa, s, false, uks ? P.rho_sb.as<double4>() : nullptr, dPb);
     if (rc != kOk) return rc;
     XcGradArgs xa{P.rho_s.as<double4>(), uks ? P.rho_sb.as<double4>() : nullptr, P.pts.as<double4>(), P.n_pts, xc,
                   P.gwx_a.as<double4>(), P.gwx_b.as<double4>(), P.gfxc.as<double>()};
