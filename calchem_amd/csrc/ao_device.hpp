@@ -86,6 +86,66 @@ __device__ __forceinline__ double2 shell_radial(const DevShell& s, const double2
     return make_double2(r0, r1);
 }
 
+// e^{-t} for t >= 0, table-driven (P. T. P. Tang, "Table-driven
+// implementation of the exponential function in IEEE floating-point
+// arithmetic", ACM TOMS 15 (1989) 144-157): t = (64 m + j) ln2/64 + r with
+// |r| <= ln2/128 (Cody-Waite reduction with ln2/64 in two parts; the first
+// product is exact under the FMA), e^{-t} = 2^-m 2^{-j/64} e^{-r}, e^{-r}
+// from its degree-5 Taylor polynomial (truncation below 4e-17 relative).
+// tab holds 2^{-j/64}, j = 0..63 (kExpNegTab, staged in LDS by the caller).
+// Within 2 ulp of the correctly rounded value (20 M samples on [0, 708]);
+// 0 for t > 708, where e^{-t} is below the smallest normal double.
+__device__ constexpr double kExpNegTab[64] = {
+    0x1.0000000000000p+0, 0x1.fa7c1819e90d8p-1, 0x1.f50765b6e4540p-1, 0x1.efa1bee615a27p-1,
+    0x1.ea4afa2a490dap-1, 0x1.e502ee78b3ff6p-1, 0x1.dfc97337b9b5fp-1, 0x1.da9e603db3285p-1,
+    0x1.d5818dcfba487p-1, 0x1.d072d4a07897cp-1, 0x1.cb720dcef9069p-1, 0x1.c67f12e57d14bp-1,
+    0x1.c199bdd85529cp-1, 0x1.bcc1e904bc1d2p-1, 0x1.b7f76f2fb5e47p-1, 0x1.b33a2b84f15fbp-1,
+    0x1.ae89f995ad3adp-1, 0x1.a9e6b5579fdbfp-1, 0x1.a5503b23e255dp-1, 0x1.a0c667b5de565p-1,
+    0x1.9c49182a3f090p-1, 0x1.97d829fde4e50p-1, 0x1.93737b0cdc5e5p-1, 0x1.8f1ae99157736p-1,
+    0x1.8ace5422aa0dbp-1, 0x1.868d99b4492edp-1, 0x1.82589994cce13p-1, 0x1.7e2f336cf4e62p-1,
+    0x1.7a11473eb0187p-1, 0x1.75feb564267c9p-1, 0x1.71f75e8ec5f74p-1, 0x1.6dfb23c651a2fp-1,
+    0x1.6a09e667f3bcdp-1, 0x1.6623882552225p-1, 0x1.6247eb03a5585p-1, 0x1.5e76f15ad2148p-1,
+    0x1.5ab07dd485429p-1, 0x1.56f4736b527dap-1, 0x1.5342b569d4f82p-1, 0x1.4f9b2769d2ca7p-1,
+    0x1.4bfdad5362a27p-1, 0x1.486a2b5c13cd0p-1, 0x1.44e086061892dp-1, 0x1.4160a21f72e2ap-1,
+    0x1.3dea64c123422p-1, 0x1.3a7db34e59ff7p-1, 0x1.371a7373aa9cbp-1, 0x1.33c08b26416ffp-1,
+    0x1.306fe0a31b715p-1, 0x1.2d285a6e4030bp-1, 0x1.29e9df51fdee1p-1, 0x1.26b4565e27cddp-1,
+    0x1.2387a6e756238p-1, 0x1.2063b88628cd6p-1, 0x1.1d4873168b9aap-1, 0x1.1a35beb6fcb75p-1,
+    0x1.172b83c7d517bp-1, 0x1.1429aaea92de0p-1, 0x1.11301d0125b51p-1, 0x1.0e3ec32d3d1a2p-1,
+    0x1.0b5586cf9890fp-1, 0x1.0874518759bc8p-1, 0x1.059b0d3158574p-1, 0x1.02c9a3e778061p-1,
+};
+
+__device__ __forceinline__ double exp_neg_tab(double t, const double* __restrict__ tab) {
+    const double kd = rint(t * 92.33248261689366);  // 64 / ln 2
+    const double rh = fma(-kd, 0.010830424696249145, t);
+    const double r = fma(-kd, 3.623510646634843e-19, rh);
+    const int k = static_cast<int>(kd);
+    double p = fma(r, -1.0 / 120.0, 1.0 / 24.0);
+    p = fma(r, p, -1.0 / 6.0);
+    p = fma(r, p, 0.5);
+    p = fma(r, p, -1.0);
+    p = fma(r, p, 1.0);
+    const double v = ldexp(tab[k & 63] * p, -(k >> 6));
+    return t <= 708.0 ? v : 0.0;
+}
+
+// shell_radial with the table-driven exponential.
+__device__ __forceinline__ double2 shell_radial_tab(const DevShell& s, const double2* __restrict__ prims, double x,
+                                                    double y, double z, const double* __restrict__ tab) {
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    double r0 = 0.0, r1 = 0.0;
+    if (r2 <= s.r2cut) {
+        for (int p = 0; p < s.n_prims; ++p) {
+            const double2 pr = prims[s.prim_start + p];
+            const double e = pr.y * exp_neg_tab(pr.x * r2, tab);
+            r0 += e;
+            r1 -= 2.0 * pr.x * e;
+        }
+        if (fabs(r0) < 1e-18 && fabs(r1) < 1e-18) r0 = r1 = 0.0;
+    }
+    return make_double2(r0, r1);
+}
+
 // Straight-line: the unscaled values u are formed first and the drop test
 // |scale| max|u| > 1e-16 (the same test on the scaled values, since scaling
 // by |scale| is monotonic) selects the scale or zero.  Zero radial sums give

@@ -74,11 +74,12 @@ struct Vec4<float> {
 // rho stage
 // ---------------------------------------------------------------------------
 // Dynamic LDS of one rho workgroup, carved in this order (every piece is a
-// multiple of 16 bytes): points, per-wave sums, radial table [shell][point],
+// multiple of 16 bytes): points, per-wave sums, the exponential's table,
+// radial table [shell][point],
 // rows, shells, primitives (the block's packed records, copied as one run),
 // FP64 leading-row values, row values.
 __host__ __device__ constexpr size_t rho_lds_fixed(int ns) {
-    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * ns * kRhoPts;
+    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * ns * kRhoPts + sizeof(double) * 64;
 }
 
 // Workgroups are dispatched round robin over the XCDs (workgroup i to XCD
@@ -121,6 +122,8 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     q += sizeof(double4) * kRhoSub * kRhoPts;
     double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 NS sums][PTS]
     q += sizeof(double) * 16 * NS * PTS;
+    double* sExp = reinterpret_cast<double*>(q);  // [64] the exponential's table
+    q += sizeof(double) * 64;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
     q += sizeof(double2) * nsh * PTS;
     RowL* sRow = reinterpret_cast<RowL*>(q);
@@ -165,6 +168,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         uint4* dst = reinterpret_cast<uint4*>(sRow);
         const int n16 = mp + nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + blk.n_prim;
         for (int i = tid; i < n16; i += kThreads) dst[i] = src[i];
+        if (tid < 64) sExp[tid] = kExpNegTab[tid];
     }
     // The workgroup's points: kRhoSub sub-tiles of 16, taken one after the
     // other with the block's records staged once.
@@ -186,7 +190,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             double2 rr = make_double2(0.0, 0.0);
             if (p < np) {
                 const double4 x = sPt[p];
-                rr = shell_radial(sSh[sh], sPrim, x.x, x.y, x.z);
+                rr = shell_radial_tab(sSh[sh], sPrim, x.x, x.y, x.z, sExp);
             }
             sRad[idx] = rr;
         }
