@@ -91,7 +91,8 @@ struct Plan {
         group_rows,
         group_meta, red_part, group_ao,
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
-        V, Vb_host, rho, rho_s, psub64, psub32, psub64b, psub32b, rho_sb, wv_b, mols, ao_mol, xc_wg, shell_atom,
+        V, Vb_host, rho, rho_s, psub64, psub32, psub64b, psub32b, class_blocks, rho_sb, wv_b, mols, ao_mol, xc_wg,
+        shell_atom,
         atom_xyz, inv_r, asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
     int n_atoms = 0;  // atoms set by calchem_hip_vxc_set_atoms (0 = not set)
     bool uks_ready = false;  // rho_sb / wv_b / psub64b / psub32b allocated (first unrestricted build)
@@ -122,7 +123,7 @@ struct Plan {
                              &block_sh, &block_meta, &group_meta, &red_part, &rho_tiles, &grad_tiles, &groups, &group_rows, &group_ao, &group_sh, &group_list,
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
-                             &psub64b, &psub32b, &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
+                             &psub64b, &psub32b, &class_blocks, &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
                              &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
@@ -503,6 +504,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     // block first; a block's tiles stay consecutive.  Cost of a tile: m_pad^2
     // plus a fixed part for its staging and AO work (that of m_pad = 128).
     std::vector<int2> htiles, hgtiles;  // rho workgroups; 16-point tiles of the gradient kernel
+    std::vector<int> hclsblk;           // blocks in class order (each class gathers its own)
     {
         const int ncls = std::max((P.max_mpad_rho + 63) / 64, 1);
         std::vector<std::vector<int>> bycls(ncls);
@@ -536,6 +538,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                 seg[x].push_back(b);
                 load[x] += block_cost(b);
             }
+            cls[c].block_begin = static_cast<int>(hclsblk.size());
+            cls[c].n_blocks = static_cast<int>(bl.size());
+            hclsblk.insert(hclsblk.end(), bl.begin(), bl.end());
             cls[c].tile_begin = static_cast<int>(htiles.size());
             for (int x = 0; x < kXcds; ++x) {
                 cls[c].seg[x] = static_cast<int>(htiles.size()) - cls[c].tile_begin;
@@ -714,7 +719,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
     ok = upload(P.perm, hperm) && upload(P.blocks, hb) && upload(P.block_rows, hrows) && upload(P.block_sh, hsh) &&
          upload(P.block_meta, hmeta) &&
-         upload(P.rho_tiles, htiles) && upload(P.grad_tiles, hgtiles) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
+         upload(P.rho_tiles, htiles) && upload(P.grad_tiles, hgtiles) && upload(P.class_blocks, hclsblk) && upload(P.group_ao, hgao) && upload(P.shell_atom, hshatom) &&
          upload(P.groups, hg) && upload(P.group_rows, hgrows) &&
          upload(P.group_sh, hgsh) && upload(P.group_meta, hgmeta) &&
          (P.reduce_segments == 1 ||
@@ -796,42 +801,54 @@ int fan_out(Plan& P, hipStream_t s, int n, F fn) {
 int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s, bool gather_mark,
             double4* sorted_b = nullptr, const double* dPb = nullptr) {
     const bool two = dPb != nullptr;
-    GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP, P.psub64.as<double>(),
-                  mode == kRhoMixed ? P.psub32.as<float>() : nullptr, mode == kRhoMixed ? 1 : 0};
-    if (launch_gather_p(ga, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
-    if (two) {
+    for (size_t c = 0; c < P.classes.size(); ++c)
+        if (!P.class_fits[mode][c]) return kLimit;
+    // The density blocks are gathered class by class, each on the stream of
+    // its class's rho launch, so one class's gather (memory-bound) runs beside
+    // another class's contraction.  With profiling marks on they are gathered
+    // in one launch first instead, so that the marks time the gather and the
+    // rho kernels apart.
+    auto gather = [&](const int* list, int n_blk, int max_mpad, hipStream_t st) {
+        GatherArgs ga{P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.mols.as<MolInfo>(), dP,
+                      P.psub64.as<double>(), mode == kRhoMixed ? P.psub32.as<float>() : nullptr,
+                      mode == kRhoMixed ? 1 : 0, list};
+        if (launch_gather_p(ga, n_blk, max_mpad, st) != hipSuccess) return false;
+        if (!two) return true;
         GatherArgs gb = ga;
         gb.P = dPb;
         gb.out64 = P.psub64b.as<double>();
         gb.out32 = mode == kRhoMixed ? P.psub32b.as<float>() : nullptr;
-        if (launch_gather_p(gb, P.n_blocks, P.max_mpad_rho, s) != hipSuccess) return kLaunchFail;
-    }
+        return launch_gather_p(gb, n_blk, max_mpad, st) == hipSuccess;
+    };
+    const bool per_class = !P.prof.on && P.classes.size() > 1;
+    if (!per_class && !gather(nullptr, P.n_blocks, P.max_mpad_rho, s)) return kLaunchFail;
     if (gather_mark) mark(P, s);
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
-    for (size_t c = 0; c < P.classes.size(); ++c)
-        if (!P.class_fits[mode][c]) return kLimit;
     // the LDS classes cover disjoint tiles: side by side (their tails overlap)
     return fan_out(P, s, static_cast<int>(P.classes.size()), [&](int c, hipStream_t st) {
+        const RhoClass& cl = P.classes[c];
+        if (per_class && !gather(P.class_blocks.as<int>() + cl.block_begin, cl.n_blocks, cl.max_mpad, st))
+            return kLaunchFail;
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
                    P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
                    P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(),
                    sorted ? 0 : 1};
-        if (!two) return launch_rho(ra, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
+        if (!two) return launch_rho(ra, mode, cl, st) == hipSuccess ? kOk : kLaunchFail;
         // both spins from one AO evaluation when the class's LDS allows,
         // else one pass per spin
-        if (rho_lds_bytes(mode, P.classes[c], 2) <= P.lds_limit) {
+        if (rho_lds_bytes(mode, cl, 2) <= P.lds_limit) {
             ra.P64b = P.psub64b.as<double>();
             ra.PSb = mode == kRhoMixed ? static_cast<const void*>(P.psub32b.p) : P.psub64b.p;
             ra.rho_out_b = sorted_b;
-            return launch_rho(ra, mode, P.classes[c], st, 2) == hipSuccess ? kOk : kLaunchFail;
+            return launch_rho(ra, mode, cl, st, 2) == hipSuccess ? kOk : kLaunchFail;
         }
-        if (launch_rho(ra, mode, P.classes[c], st) != hipSuccess) return kLaunchFail;
+        if (launch_rho(ra, mode, cl, st) != hipSuccess) return kLaunchFail;
         RhoArgs rb = ra;
         rb.P64 = P.psub64b.as<double>();
         rb.PS = mode == kRhoMixed ? static_cast<const void*>(P.psub32b.p) : P.psub64b.p;
         rb.rho_out = sorted_b;
-        return launch_rho(rb, mode, P.classes[c], st) == hipSuccess ? kOk : kLaunchFail;
+        return launch_rho(rb, mode, cl, st) == hipSuccess ? kOk : kLaunchFail;
     });
 }
 

@@ -116,6 +116,7 @@ struct RhoClass {
     int grad_tile_begin, grad_n_tiles;  // range in the 16-point tile list of the XC gradient's AO kernel
     int max_mpad, max_nsh, max_m64, max_prim;
     int seg[kXcds + 1];
+    int block_begin, n_blocks;  // the class's blocks: range in the plan's class-ordered block list
 };
 
 struct VArgs {
@@ -145,6 +146,7 @@ struct GatherArgs {
     double* out64;          // may be null
     float* out32;           // may be null
     int out64_leading_only; // write out64 only for the block's first m64 columns
+    const int* block_list;  // workgroup w gathers block block_list[w] (null: block w)
 };
 
 // One (AO mu, group holding mu) entry of the V reduction: where row li(mu)

@@ -1117,7 +1117,7 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
 // of the launch).
 __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
     extern __shared__ int sIdx[];
-    const RhoBlock B = a.blocks[blockIdx.x];
+    const RhoBlock B = a.blocks[a.block_list ? a.block_list[blockIdx.x] : static_cast<int>(blockIdx.x)];
     const MolInfo mol = a.mols[B.mol];
     const double* __restrict__ P = a.P + mol.p_off;
     const int mp = B.m_pad, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
