@@ -1,4 +1,5 @@
-"""Edge cases the reference's own engine tests hold, on the GPU through the C ABI.
+"""Edge cases the reference's own engine tests hold, on the GPU through the C ABI,
+and invariances of the build paths.
 
 Reference checks followed: a grid of one zero-weight point gives E_xc = 0 and
 V = 0 exactly for both the restricted and the unrestricted engine, and finite
@@ -69,3 +70,30 @@ def test_measured_mfma_ceiling_is_plausible(prec, datasheet):
     import bench
     t = bench.measured_mfma_peak(prec, n_wg=1024, iters=500)
     assert 0.4 * datasheet < t <= 1.05 * datasheet, t
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_profiled_builds_equal_plain_builds(prec):
+    # With profiling marks on, a build gathers all density blocks in one launch
+    # before the rho launches; without, each rho launch class gathers its own
+    # blocks on its side stream.  Same kernels on the same blocks: results are
+    # bit-identical (restricted and unrestricted).
+    w = configs.WORKLOADS["c8h18"]
+    mol = w["build"]()
+    basis = BasisSet.from_name(w["basis"], mol)
+    grid = MolecularGrid.explicit(mol, 50, 194)
+    P = configs.random_density(basis, w["n_occ"], 3)
+    eng = RksVxcEngine(basis, grid)
+    assert eng.stats()["max_m_rho"] > 64  # blocks in more than one 64-row launch class
+    e0, V0, n0 = eng.evaluate_full("b3lyp", P, prec)
+    eng.set_profiling(True)
+    e1, V1, n1 = eng.evaluate_full("b3lyp", P, prec)
+    eng.set_profiling(False)
+    assert e0 == e1 and n0 == n1 and np.array_equal(V0, V1)
+    ue = UksVxcEngine(basis, grid)
+    Pa, Pb = 0.6 * P, 0.4 * P
+    r0 = ue.evaluate_full("pbe", Pa, Pb, prec)
+    ue.set_profiling(True)
+    r1 = ue.evaluate_full("pbe", Pa, Pb, prec)
+    ue.set_profiling(False)
+    assert r0[0] == r1[0] and r0[3] == r1[3] and np.array_equal(r0[1], r1[1]) and np.array_equal(r0[2], r1[2])
