@@ -334,12 +334,23 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                         if (sv > 0.0) return sNZero[c] > 0 ? 0.0 : sBig[c] * (sTiny[c] / sv);
                         return sNZero[c] > 1 ? 0.0 : sBig[c] * sTiny[c];
                     };
-                    const double p_ab = except(at, s_ab);
-                    const double p_ba = except(b, s_ba);
+                    double p_ab, p_ba;
+                    if (s_ab > 1e-12 && s_ba > 1e-12) {
+                        // both leave-one-out products by division, from one reciprocal
+                        const double r = recip(s_ab * s_ba);
+                        p_ab = sCell[at] * (s_ba * r);
+                        p_ba = sCell[b] * (s_ab * r);
+                    } else {
+                        p_ab = except(at, s_ab);
+                        p_ba = except(b, s_ba);
+                    }
+                    // dsab_c = slope (dra_c - mu rv_c / R) / R, with t = slope / R and
+                    // u = mu / R formed once per pair
+                    const double t = slope * inv, u = mu * inv, dp = p_ab - p_ba;
 #pragma unroll
                     for (int c = 0; c < 3; ++c) {
-                        const double dsab = slope * (dra[c] - mu * rv[c] * inv) * inv;
-                        dzs[c] += p_ab * dsab - p_ba * dsab;
+                        const double dsab = t * fma(-u, rv[c], dra[c]);
+                        dzs[c] += dp * dsab;
                         if (b == host) dph[c] -= p_ba * dsab;
                     }
                 }
