@@ -638,7 +638,8 @@ constexpr int kVSlots = 4;
 // 16-byte alignment after it.
 template <typename T, int MAXT>
 __host__ __device__ constexpr size_t vgroup_lds_fixed() {
-    return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+    return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(double) * 64 +
+           sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
            sizeof(RowL) * MAXT * 16 + sizeof(T) * 2 * MAXT * 16 * vgroup_ld<T>();
 }
 
@@ -659,6 +660,8 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     double4* sPt = reinterpret_cast<double4*>(q);  // [kVSlots][kVPts] points, then [kVSlots][kVPts] wv
     double4* sWv = sPt + kVSlots * kVPts;
     q += sizeof(double4) * 2 * kVSlots * kVPts;
+    double* sExp = reinterpret_cast<double*>(q);  // [64] the exponential's table
+    q += sizeof(double) * 64;
     int2* sPair = reinterpret_cast<int2*>(q);
     q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
@@ -682,6 +685,7 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
         for (int i = tid; i < mp; i += NT) reinterpret_cast<uint4*>(sRow)[i] = src[i];
         const int nrest = nsh * static_cast<int>(sizeof(DevShell) / sizeof(uint4)) + G.n_prim;
         for (int i = tid; i < nrest; i += NT) reinterpret_cast<uint4*>(sSh)[i] = src[mp + i];
+        if (tid < 64) sExp[tid] = kExpNegTab[tid];
     }
 
     // Pairs p = ti (ti + 1) / 2 + tj (ti >= tj) in this order; wave w takes
@@ -722,7 +726,8 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
         for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
             const double4 x = pt[p];
-            sRad[idx] = st * kVPts + p < npts ? shell_radial(sSh[sh], sPrim, x.x, x.y, x.z) : make_double2(0.0, 0.0);
+            sRad[idx] = st * kVPts + p < npts ? shell_radial_tab(sSh[sh], sPrim, x.x, x.y, x.z, sExp)
+                                              : make_double2(0.0, 0.0);
         }
     };
     auto evaluate = [&](int st) {
