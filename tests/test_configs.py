@@ -1,5 +1,6 @@
 """Workload generators: geometry rules, seeded RNG, S-orthonormal densities."""
 import math
+import os
 
 import numpy as np
 
@@ -80,3 +81,16 @@ def test_random_density_is_s_orthonormal_projector():
         # P S P = 2 P (idempotent up to the occupation 2)
         assert np.abs(P @ S @ P - 2 * P).max() < 1e-9 * max(1.0, np.abs(P).max())
         assert np.linalg.eigvalsh(P).min() > -1e-10
+
+
+def test_bench_refuses_more_than_one_gpu():
+    # bench.py measures one GPU: any other --gpus exits 2 with a message and
+    # prints no JSON line (no device work happens before the check)
+    import subprocess
+    import sys as _sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([_sys.executable, os.path.join(root, "bench.py"), "--gpus", "8"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and r.stdout.strip() == ""
