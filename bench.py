@@ -279,7 +279,7 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the Vxc build runs on one "
-              "MI355X; multi-GPU sharding is described in DESIGN.md and not run.", file=sys.stderr)
+              "MI355X; sharding the grid blocks over several GPUs is not run here.", file=sys.stderr)
         return 2
 
     from calchem_amd import configs, hip_runtime as hr
