@@ -389,6 +389,9 @@ def main() -> int:
     best = "fp32" if gate and r32["ms"] < r64["ms"] else "fp64"
     rb = results[best]
     log(f"fp32 vs fp64: dE={dE:.3e} max|dV|={dV32:.3e} dN={dN:.3e} gate={'pass' if gate else 'FAIL'}")
+    if not gate:
+        log(f"mixed build fails the |dE_xc| < {E_GATE:g} Eh gate against the FP64 build: no result line")
+        return 3
 
     # Dominant kernel of the reported path and its roofline.
     st = rb["stats"]
@@ -425,6 +428,10 @@ def main() -> int:
         parity = {"points": int(idx.size), "dE_xc": e_g - cb["exc"],
                   "max_dV": float(np.max(np.abs(V_g - cb["V"]))), "dN": n_g - cb["nelec"]}
         log(f"FP64 GPU vs oracle on the sample: {parity}")
+        if not (abs(parity["dE_xc"]) < 1e-10 and parity["max_dV"] < 1e-9 and abs(parity["dN"]) < 1e-9):
+            log("FP64 GPU build disagrees with the oracle (bars |dE_xc| < 1e-10, max|dV| < 1e-9, |dN| < 1e-9): "
+                "no result line")
+            return 4
 
     out = {
         "metric": METRIC,

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcalchem_hip.so")
 PROBE_LIB = os.path.join(HERE, "libcalchem_probe.so")
 PROBE_SOURCE = "probe_mfma.hip"
-SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip"]
+SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip", "xc_points.hip"]
 ARCH = os.environ.get("CALCHEM_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

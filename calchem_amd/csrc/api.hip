@@ -237,7 +237,8 @@ struct MolSpec {
 
 int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr, int n_prims,
                 const CalchemAoComp* cp, int n_comps, const double* xyzw, int stride, int n_pts,
-                const std::vector<MolSpec>& mspec, bool has_w, const CalchemVxcOptions* opt, uint64_t id) {
+                const std::vector<MolSpec>& mspec, bool has_w, const CalchemVxcOptions* opt, uint64_t id,
+                bool reuse_ok) {
     if (!sh || !pr || !cp || !xyzw || n_shells <= 0 || n_prims <= 0 || n_comps <= 0 || n_pts <= 0 ||
         mspec.empty() || id == 0)
         return kBadArgs;
@@ -260,8 +261,11 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
         auto it = g_table.find(id);
+        // upload_grid keeps the reference's reuse rule (same id and sizes: ok,
+        // cuda_kernels.cu:7382-7386); create / create_batched are per-geometry
+        // set-up, so an id in use must be released first.
         if (it != g_table.end())
-            return (it->second->n_in == n_pts && it->second->n_ao == n_ao && it->second->n_mol == n_mol)
+            return (reuse_ok && it->second->n_in == n_pts && it->second->n_ao == n_ao && it->second->n_mol == n_mol)
                        ? kOk
                        : kMismatch;
     }
@@ -989,7 +993,7 @@ int calchem_hip_vxc_upload_grid(const CalchemAoShell* shells, int n_shells, cons
     o.memory_limit_bytes = memory_limit_bytes;
     const std::vector<MolSpec> one{MolSpec{0, n_shells, 0, n_pts, n_ao}};
     return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyz.data(), 4, n_pts, one, false, &o,
-                       grid_id);
+                       grid_id, true);
 }
 
 int calchem_hip_vxc_eval_rho(uint64_t grid_id, const double* P_mat, double* rho_grad, int n_pts, int n_ao) {
@@ -1053,7 +1057,8 @@ int calchem_hip_vxc_create(const CalchemAoShell* shells, int n_shells, const Cal
                            const CalchemVxcOptions* opts, uint64_t handle) {
     if (n_shells <= 0 || n_pts <= 0 || n_ao <= 0) return kBadArgs;
     const std::vector<MolSpec> one{MolSpec{0, n_shells, 0, n_pts, n_ao}};
-    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyzw, 4, n_pts, one, true, opts, handle);
+    return create_plan(shells, n_shells, prims, n_prims, comps, n_comps, xyzw, 4, n_pts, one, true, opts, handle,
+                       false);
 }
 
 int calchem_hip_vxc_create_batched(int n_mol, const int* mol_shell_begin, const int* mol_point_begin,
@@ -1066,7 +1071,7 @@ int calchem_hip_vxc_create_batched(int n_mol, const int* mol_shell_begin, const 
         ms[k] = MolSpec{mol_shell_begin[k], mol_shell_begin[k + 1], mol_point_begin[k], mol_point_begin[k + 1],
                         mol_n_ao[k]};
     return create_plan(shells, mol_shell_begin[n_mol], prims, n_prims, comps, n_comps, xyzw, 4,
-                       mol_point_begin[n_mol], ms, true, opts, handle);
+                       mol_point_begin[n_mol], ms, true, opts, handle, false);
 }
 
 int calchem_hip_vxc_build(uint64_t handle, int xc, int precision, const double* P_mat, double* V_out, double* exc,
@@ -1148,6 +1153,7 @@ int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, con
     auto plan = find_plan(handle);
     if (!plan) return kMismatch;
     Plan& P = *plan;
+    if (P.n_mol != 1) return kMismatch;  // unrestricted builds: single-molecule plans only
     std::lock_guard<std::mutex> lk(P.mu);
     hipSetDevice(P.device);
     return run_build_uks(P, xc, precision, d_Pa, d_Pb, d_Va, d_Vb, d_exc_nelec, static_cast<hipStream_t>(stream));

@@ -140,7 +140,8 @@ int calchem_hip_vxc_release(uint64_t grid_id);
  * row-major) in; Va, Vb (full symmetric), E_xc and N = sum_g w_g (rho_a + rho_b)
  * out.  Same plan as the restricted build; the first call allocates the
  * second spin's per-point buffers.  _device: device pointers, asynchronous
- * on `stream`, d_exc_nelec[2] = (E_xc, N). */
+ * on `stream`, d_exc_nelec[2] = (E_xc, N).  Single-molecule plans only (-2
+ * for a batched plan). */
 int calchem_hip_vxc_build_uks(uint64_t handle, int xc, int precision, const double *Pa, const double *Pb,
                               double *Va_out, double *Vb_out, double *exc, double *nelec);
 int calchem_hip_vxc_build_uks_device(uint64_t handle, int xc, int precision, const double *d_Pa,
@@ -197,7 +198,8 @@ int calchem_hip_vxc_xc_gradient(uint64_t handle, int xc, const double *Pa, const
 /* ---- Fused build (functional on the device) ----
  * Replaces the per-geometry setup of RksVxcEngine::new_with_gpu
  * (crates/calchem-dft/src/vxc.rs:429-595): xyzw[n_pts*4] = (x, y, z, weight)
- * of MolecularGrid.points (grid.rs:33-51). */
+ * of MolecularGrid.points (grid.rs:33-51).  A handle already in use returns -2
+ * (release it first: a plan is one geometry); so does create_batched. */
 int calchem_hip_vxc_create(const CalchemAoShell *shells, int n_shells, const CalchemAoPrim *prims, int n_prims,
                            const CalchemAoComp *comps, int n_comps, const double *xyzw, int n_pts, int n_ao,
                            const CalchemVxcOptions *opts, uint64_t handle);
@@ -227,6 +229,22 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats *out);
  * -6 when max_out is too small. */
 long calchem_hip_grid_build(int n_atoms, const int *z, const double *xyz_bohr, int level, int n_rad, int n_ang,
                             int radial_min, double *out, long max_out);
+
+/* ---- The functional on caller points ----
+ * No reference entry point: evaluates eval_xc_spin (crates/calchem-dft/src/xc.rs:159-223)
+ * with the device code the builds use, at arbitrary spin invariants, so that
+ * its branches can be checked against fixed values (the reference's own
+ * checks at xc.rs:1212-1247 call eval_xc_spin directly).
+ * in[n*5] = (rho_a, rho_b, gamma_aa, gamma_ab, gamma_bb) per point; w[n] the
+ * weights (NULL = 1); out[n*6] = (f, df/drho_a, df/drho_b, df/dgamma_aa,
+ * df/dgamma_ab, df/dgamma_bb); sums[2] = (sum w f, sum w (rho_a + rho_b))
+ * summed in a fixed order, or NULL.  _device: device pointers, enqueued on
+ * `stream` (a hipStream_t, NULL = default stream); the per-workgroup partial
+ * sums of a call are stream-ordered allocations of that call.  The host form
+ * is synchronous. */
+int calchem_hip_xc_eval(int xc, long long n, const double *in, const double *w, double *out, double *sums);
+int calchem_hip_xc_eval_device(int xc, long long n, const double *d_in, const double *d_w, double *d_out,
+                               double *d_sums, void *stream);
 
 #ifdef __cplusplus
 }

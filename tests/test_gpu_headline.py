@@ -1,0 +1,40 @@
+"""The benchmarked workload itself under test: 20 waters, 6-31G(d,p) (500 AO),
+75 x 302 Becke grid (1,147,146 points), B3LYP, default block and group shapes
+(the k_vxc_group / k_vxc_group_pipe path the throughput figure times).
+
+FP64 build against the oracle on every point: |dE_xc| < 1e-10 Eh,
+max|dV| < 1e-9, |dN| < 1e-9 and exact symmetry; the mixed build against the
+FP64 build: |dE_xc| < 1e-7 Eh (the mixed-precision gate).
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from calchem_amd import configs
+from calchem_amd.basis import BasisSet
+from calchem_amd.grid import MolecularGrid
+from calchem_amd.vxc import RksVxcEngine
+
+pytestmark = pytest.mark.gpu
+
+
+def test_water20_75x302_default_shapes():
+    w = configs.WORKLOADS["water20"]
+    mol = w["build"]()
+    basis = BasisSet.from_name(w["basis"], mol)
+    assert basis.n_ao == 500
+    P = configs.random_density(basis, w["n_occ"], w["seed"])
+    grid = MolecularGrid.explicit(mol, 75, 302)
+    print(f"20 waters: {grid.n_points} points", flush=True)
+    eng = RksVxcEngine(basis, grid)
+    st = eng.stats()
+    assert st["n_v_pairs"] == 0  # every group register-resident: the default V kernels
+    e, V, ne = eng.evaluate_full("b3lyp", P, "fp64")
+    e32, V32, n32 = eng.evaluate_full("b3lyp", P, "fp32")
+    e_ref, V_ref, ne_ref = O.vxc_rks(O.OracleBasis(basis), grid.points, "b3lyp", P)
+    dv = float(np.abs(V - V_ref).max())
+    print(f"fp64 vs oracle: dE={e - e_ref:.2e} max|dV|={dv:.2e} dN={ne - ne_ref:.2e}; "
+          f"mixed vs fp64: dE={e32 - e:.2e} max|dV|={np.abs(V32 - V).max():.2e} dN={n32 - ne:.2e}")
+    assert abs(e - e_ref) < 1e-10 and dv < 1e-9 and abs(ne - ne_ref) < 1e-9
+    assert np.abs(V - V.T).max() == 0.0 and np.abs(V32 - V32.T).max() == 0.0
+    assert abs(e32 - e) < 1e-7

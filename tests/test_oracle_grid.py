@@ -117,3 +117,82 @@ def test_explicit_grid_raw_count_and_weights():
     g = O.build_grid(mol, "explicit", 75, 302)
     assert g.shape[0] <= 3 * 75 * 302
     assert np.all(g[:, 3] <= g[:, 4] * (1 + 1e-15))
+
+
+# ---------------------------------------------------------------------------
+# The Becke partition and its radii, pinned independently of the oracle's own
+# code: Becke, J. Chem. Phys. 88, 2547 (1988), eqs. 11-13, 19 and Appendix
+# (k = 3 iterations of p(x) = 3x/2 - x^3/2, atomic-size adjustment
+# a_ij = u_ij / (u_ij^2 - 1), u_ij = (chi - 1)/(chi + 1), chi = R_i / R_j,
+# |a_ij| <= 1/2), with the Bragg-Slater radii of Slater, J. Chem. Phys. 41,
+# 3199 (1964) (C 0.70, O 0.60 Angstrom) and Becke's 0.35 Angstrom for H.
+# The reference restates the same construction at grid.rs:58-115,321-419.
+# ---------------------------------------------------------------------------
+ANG = 1.8897261245650618
+SLATER_1964_ANGSTROM = {1: 0.35, 6: 0.70, 8: 0.60}
+
+
+def becke1988_weights(z, xyz, r):
+    """Unscreened Becke cell weights w_A(r) for every atom A (numpy, from the paper)."""
+    n = len(z)
+    rad = np.array([SLATER_1964_ANGSTROM[int(q)] * ANG for q in z])
+    d = np.linalg.norm(r[None, :] - xyz, axis=1)
+    cell = np.ones(n)
+    for i in range(n):
+        for j in range(n):
+            if i == j:
+                continue
+            mu = (d[i] - d[j]) / np.linalg.norm(xyz[i] - xyz[j])
+            chi = rad[i] / rad[j]
+            u = (chi - 1.0) / (chi + 1.0)
+            a = min(0.5, max(-0.5, u / (u * u - 1.0)))
+            f = mu + a * (1.0 - mu * mu)
+            for _ in range(3):
+                f = 1.5 * f - 0.5 * f ** 3
+            cell[i] *= 0.5 * (1.0 - f)
+    return cell / cell.sum()
+
+
+def test_bragg_slater_radii_of_the_workload_elements():
+    for zz, ang in SLATER_1964_ANGSTROM.items():
+        assert O.lib().or_bragg_slater_bohr(zz) == pytest.approx(ang * ANG, rel=1e-15), zz
+
+
+@pytest.mark.parametrize("which", ["water", "c8h18"])
+def test_becke_weights_match_unscreened_becke_1988(which):
+    from calchem_amd import configs
+    mol = distorted_water_grid() if which == "water" else configs.alkane_c8h18()
+    z = mol.z_array()
+    xyz = mol.coords_bohr()
+    rng = np.random.default_rng(3)
+    g = O.build_grid(mol, "explicit", 30, 110)
+    # points of the grid itself (where weights matter) plus random points around the atoms
+    pts = np.vstack([g[rng.choice(g.shape[0], 300, replace=False), :3],
+                     xyz[rng.integers(0, len(z), 300)] + rng.normal(scale=1.5, size=(300, 3))])
+    worst = 0.0
+    for r in pts:
+        ref = becke1988_weights(z, xyz, r)
+        got = np.array([O.becke_weight(mol, h, r) for h in range(len(z))])
+        worst = max(worst, float(np.abs(got - ref).max()))
+    # the screened form drops cell products below 1e-20 and cells with 1 - s < 1e-17
+    assert worst < 1e-14, worst
+
+
+@pytest.mark.parametrize("heavy,frac", [(8, 0.35 / (0.35 + 0.60)), (6, 0.35 / (0.35 + 0.70)), (1, 0.5)])
+def test_becke_cell_boundary_sits_at_the_radius_ratio(heavy, frac):
+    """On the axis of a diatomic H-X, w_H = 1/2 where r_H / r_X = R_H / R_X
+    (nu = 0 at mu = u_HX, Becke 1988 eq. A2): the H cell is the smaller one
+    beside O or C, which fixes the sign of a_ij and the radii's ratio."""
+    L = 1.9
+    mol = Molecule([Atom.from_bohr("H", (0.0, 0.0, 0.0)), Atom.from_bohr(
+        {8: "O", 6: "C", 1: "H"}[heavy], (0.0, 0.0, L))])
+    lo, hi = 1e-6, L - 1e-6
+    for _ in range(200):
+        mid = 0.5 * (lo + hi)
+        if O.becke_weight(mol, 0, np.array([0.0, 0.0, mid])) > 0.5:
+            lo = mid
+        else:
+            hi = mid
+    assert abs(0.5 * (lo + hi) / L - frac) < 1e-12, (heavy, 0.5 * (lo + hi) / L, frac)
+    if heavy != 1:
+        assert frac < 0.5
