@@ -59,6 +59,7 @@ EXPORTS = [
     "calchem_hip_vxc_set_atoms", "calchem_hip_vxc_xc_gradient",
     "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
     "calchem_hip_grid_build", "calchem_hip_xc_eval", "calchem_hip_xc_eval_device",
+    "calchem_hip_xc_eval_device_noalloc",
 ]
 
 _lib = None
@@ -103,6 +104,7 @@ def load() -> C.CDLL:
     L.calchem_hip_grid_build.argtypes = [i32, vp, vp, i32, i32, i32, i32, vp, C.c_long]
     L.calchem_hip_xc_eval.argtypes = [i32, C.c_longlong, vp, vp, vp, vp]
     L.calchem_hip_xc_eval_device.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
+    L.calchem_hip_xc_eval_device_noalloc.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
     _lib = L
     return L
 

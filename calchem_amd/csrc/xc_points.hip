@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <mutex>
 
 #include "../../include/calchem_hip.h"
 #include "common.hpp"
@@ -148,6 +149,32 @@ int enqueue_points(int xc, long long n, const double* d_in, const double* d_w, d
 // Largest point count per call: the grid's workgroup count stays below 2^31.
 constexpr long long kMaxPoints = (1LL << 31) * kPtThreads - kPtThreads;
 
+// Scratch of the entry point that allocates nothing: the per-workgroup partial
+// sums of up to kNoAllocMaxWg workgroups (16,777,216 points), 1 MiB per GPU,
+// held until the process ends.  Every call on a GPU shares that GPU's copy.
+constexpr long long kNoAllocMaxWg = 65536;
+constexpr long long kNoAllocMaxPoints = kNoAllocMaxWg * kPtThreads;
+__device__ __align__(256) double scratch[2 * kNoAllocMaxWg];
+
+// Device address of `scratch` per GPU, looked up once per device with that
+// device current (the host-side name of the array is not a device address).
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_for_current_device(double** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return kNoDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return kNoDevice;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return kOk;
+}
+
 }  // namespace
 }  // namespace chx
 
@@ -178,6 +205,28 @@ int calchem_hip_xc_eval_device(int xc, long long n, const double* d_in, const do
     const int rc = enqueue_points(xc, n, d_in, d_w, d_out, d_sums, part, s);
     if (part && hipFreeAsync(part, s) != hipSuccess && rc == kOk) return kLaunchFail;
     return rc;
+}
+
+int calchem_hip_xc_eval_device_noalloc(int xc, long long n, const double* d_in, const double* d_w, double* d_out,
+                                       double* d_sums, void* stream) {
+    if (n < 0 || (n > 0 && (!d_in || !d_out))) return kBadArgs;
+    if (n > kNoAllocMaxPoints) return kLimit;
+    if (!pick_points_kernel(xc)) return kUnsupported;
+    if (!device_present()) return kNoDevice;
+    double* part = nullptr;
+    const int rc = scratch_for_current_device(&part);
+    if (rc != kOk) return rc;
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (n == 0) {  // the sums of no partials are zeros
+        if (d_sums) {
+            hipLaunchKernelGGL(k_sum_pairs, dim3(1), dim3(kSumThreads), 0, s, nullptr, 0LL, d_sums);
+            if (hipGetLastError() != hipSuccess) return kLaunchFail;
+        }
+        return kOk;
+    }
+    // Every partial that k_sum_pairs reads is written by this call's point
+    // kernel first, so nothing needs clearing between calls.
+    return enqueue_points(xc, n, d_in, d_w, d_out, d_sums, part, s);
 }
 
 int calchem_hip_xc_eval(int xc, long long n, const double* in, const double* w, double* out, double* sums) {

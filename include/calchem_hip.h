@@ -245,6 +245,14 @@ long calchem_hip_grid_build(int n_atoms, const int *z, const double *xyz_bohr, i
 int calchem_hip_xc_eval(int xc, long long n, const double *in, const double *w, double *out, double *sums);
 int calchem_hip_xc_eval_device(int xc, long long n, const double *d_in, const double *d_w, double *d_out,
                                double *d_sums, void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): calchem_hip_xc_eval_device without any allocation.  The
+ * partial sums live in a fixed 1 MiB device array per GPU, so n is at most
+ * 16,777,216 (-6 above).  Calls that overlap on one GPU share that array and
+ * race: issue them on one stream. */
+int calchem_hip_xc_eval_device_noalloc(int xc, long long n, const double *d_in, const double *d_w, double *d_out,
+                                       double *d_sums, void *stream);
 
 #ifdef __cplusplus
 }
