@@ -31,6 +31,7 @@ _DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "ba
 
 _NAME_TO_FILE = {
     "sto-3g": "sto-3g.json",
+    "6-31g": "6-31g.json",
     "6-31g(d)": "6-31gd.json",
     "6-31g*": "6-31gd.json",
     "6-31gd": "6-31gd.json",

@@ -1793,3 +1793,344 @@ int or_ao_dense(const or_basis_t *b, long n_pts, const double *pts, double *val,
     shells_free(&S);
     return 0;
 }
+
+/* ======================================================================== */
+/* Integrals of the SCF Hamiltonian (the SCF row beside the Vxc build).     */
+/*   Boys function F_m(T)                         boys.rs:109-192           */
+/*   overlap / kinetic / nuclear attraction        one_electron.rs:7-168,   */
+/*                                                 primitive.rs:44-174      */
+/*   electron repulsion (ab|cd): Obara-Saika VRR to [e0|f0], Head-Gordon-    */
+/*   Pople HRR to (ab|cd) after contraction        eri.rs:276-1018          */
+/* Contracted functions as in the AO code: c~_p N(alpha_p; L00) times the   */
+/* component scale N(alpha0; lmn)/N(alpha0; L00) (each Cartesian component  */
+/* normalised, calchem-basis/src/lib.rs:200-202).                           */
+/* ======================================================================== */
+
+/* F_m(T) = int_0^1 u^{2m} exp(-T u^2) du for m = 0..mmax, in long double:  */
+/* the series e^{-T} sum_k (2T)^k / ((2M+1)(2M+3)...(2M+2k+1)) for F_M and   */
+/* downward recursion below T = 45; F_0 = sqrt(pi/T) erf(sqrt T) / 2 and     */
+/* upward recursion above.                                                  */
+void or_boys(int mmax, double T, double *F) {
+    long double t = T, et = expl(-t), Fm[64];
+    if (mmax > 63) mmax = 63;
+    if (T < 45.0) {
+        long double term = 1.0L / (2 * mmax + 1), sum = term;
+        for (int k = 1; k < 4000; ++k) {
+            term *= 2.0L * t / (long double)(2 * mmax + 2 * k + 1);
+            sum += term;
+            if (term < 1e-22L * sum) break;
+        }
+        Fm[mmax] = et * sum;
+        for (int m = mmax; m > 0; --m) Fm[m - 1] = (2.0L * t * Fm[m] + et) / (long double)(2 * m - 1);
+    } else {
+        Fm[0] = 0.5L * sqrtl(3.14159265358979323846264338327950288L / t) * erfl(sqrtl(t));
+        for (int m = 0; m < mmax; ++m) Fm[m + 1] = ((long double)(2 * m + 1) * Fm[m] - et) / (2.0L * t);
+    }
+    for (int m = 0; m <= mmax; ++m) F[m] = (double)Fm[m];
+}
+
+/* Cartesian triples (x, y, z) with x + y + z <= L, indexed by total n first. */
+static int tri_index(int x, int y, int z) {
+    int n = x + y + z, r = n - x;
+    (void)y;
+    return n * (n + 1) * (n + 2) / 6 + r * (r + 1) / 2 + z;
+}
+static int tri_count(int L) { return (L + 1) * (L + 2) * (L + 3) / 6; }
+
+/* Kinetic 1D factor <d/dx a | d/dx b> in the units of os1d. */
+static double kin1d(int i, int j, double a, double b, double PA, double PB, double p) {
+    double v = 4.0 * a * b * os1d(i + 1, j + 1, PA, PB, p);
+    if (i > 0) v -= 2.0 * b * i * os1d(i - 1, j + 1, PA, PB, p);
+    if (j > 0) v -= 2.0 * a * j * os1d(i + 1, j - 1, PA, PB, p);
+    if (i > 0 && j > 0) v += (double)(i * j) * os1d(i - 1, j - 1, PA, PB, p);
+    return v;
+}
+
+/* Nuclear attraction (a|1/r_C|b) of one primitive pair by Obara-Saika VRR on
+ * Theta^(m)(e, 0) and HRR (a, b+1_i) = (a+1_i, b) + AB_i (a, b). */
+static double nuc_hrr(const double *th0, const int a[3], const int b[3], const double AB[3]) {
+    int i = b[0] > 0 ? 0 : (b[1] > 0 ? 1 : (b[2] > 0 ? 2 : -1));
+    if (i < 0) return th0[tri_index(a[0], a[1], a[2])];
+    int a1[3] = {a[0], a[1], a[2]}, b1[3] = {b[0], b[1], b[2]};
+    a1[i] += 1;
+    b1[i] -= 1;
+    return nuc_hrr(th0, a1, b1, AB) + AB[i] * nuc_hrr(th0, a, b1, AB);
+}
+
+int or_one_electron(const or_basis_t *b, int n_atoms, const int *z, const double *xyz, double *Smat,
+                    double *Tmat, double *Vmat) {
+    shells_t S;
+    if (shells_build(b, &S) != 0) return -1;
+    const int n = S.n_ao;
+    for (int s1 = 0; s1 < S.n_shells; ++s1)
+        for (int s2 = 0; s2 < S.n_shells; ++s2) {
+            const shell_t *A = &S.sh[s1], *B = &S.sh[s2];
+            const int L = A->l + B->l, ne = tri_count(L);
+            double AB[3], AB2 = 0.0;
+            for (int q = 0; q < 3; ++q) {
+                AB[q] = A->center[q] - B->center[q];
+                AB2 += AB[q] * AB[q];
+            }
+            double *th = (double *)malloc(sizeof(double) * (size_t)(L + 1) * ne);
+            double F[64];
+            for (int ia = 0; ia < A->nfn; ++ia)
+                for (int ib = 0; ib < B->nfn; ++ib) {
+                    const int *la = comp_of(A->l, ia), *lb = comp_of(B->l, ib);
+                    double ts = 0.0, tt = 0.0, tv = 0.0;
+                    for (int pa = 0; pa < A->n_prim; ++pa)
+                        for (int pb = 0; pb < B->n_prim; ++pb) {
+                            const double a = A->alpha[pa], bb = B->alpha[pb], p = a + bb;
+                            const double Kab = exp(-a * bb / p * AB2);
+                            const double cw = A->wnorm[pa] * B->wnorm[pb];
+                            double P[3], PA[3], PB[3], s1[3], k1[3];
+                            for (int q = 0; q < 3; ++q) {
+                                P[q] = (a * A->center[q] + bb * B->center[q]) / p;
+                                PA[q] = P[q] - A->center[q];
+                                PB[q] = P[q] - B->center[q];
+                                s1[q] = os1d(la[q], lb[q], PA[q], PB[q], p);
+                                k1[q] = kin1d(la[q], lb[q], a, bb, PA[q], PB[q], p);
+                            }
+                            const double pre = Kab * pow(OR_PI / p, 1.5);
+                            ts += cw * pre * s1[0] * s1[1] * s1[2];
+                            tt += cw * pre * 0.5 * (k1[0] * s1[1] * s1[2] + s1[0] * k1[1] * s1[2] + s1[0] * s1[1] * k1[2]);
+                            for (int c = 0; c < n_atoms; ++c) {
+                                double PC[3], PC2 = 0.0;
+                                for (int q = 0; q < 3; ++q) {
+                                    PC[q] = P[q] - xyz[3 * c + q];
+                                    PC2 += PC[q] * PC[q];
+                                }
+                                or_boys(L, p * PC2, F);
+                                /* th[m * ne + e] */
+                                for (int m = 0; m <= L; ++m) th[(size_t)m * ne] = 2.0 * OR_PI / p * Kab * F[m];
+                                for (int tot = 1; tot <= L; ++tot)
+                                    for (int x = tot; x >= 0; --x)
+                                        for (int y = tot - x; y >= 0; --y) {
+                                            int e[3] = {x, y, tot - x - y};
+                                            int i = e[0] > 0 ? 0 : (e[1] > 0 ? 1 : 2);
+                                            int e1[3] = {e[0], e[1], e[2]};
+                                            e1[i] -= 1;
+                                            const int ie = tri_index(e[0], e[1], e[2]), i1 = tri_index(e1[0], e1[1], e1[2]);
+                                            int i2 = -1;
+                                            if (e1[i] > 0) {
+                                                int e2[3] = {e1[0], e1[1], e1[2]};
+                                                e2[i] -= 1;
+                                                i2 = tri_index(e2[0], e2[1], e2[2]);
+                                            }
+                                            for (int m = 0; m <= L - tot; ++m) {
+                                                double v = PA[i] * th[(size_t)m * ne + i1] - PC[i] * th[(size_t)(m + 1) * ne + i1];
+                                                if (i2 >= 0)
+                                                    v += e1[i] / (2.0 * p) * (th[(size_t)m * ne + i2] - th[(size_t)(m + 1) * ne + i2]);
+                                                th[(size_t)m * ne + ie] = v;
+                                            }
+                                        }
+                                tv -= z[c] * cw * nuc_hrr(th, la, lb, AB);
+                            }
+                        }
+                    const size_t o = (size_t)(A->ao_offset + ia) * n + B->ao_offset + ib;
+                    const double sc = A->scale[ia] * B->scale[ib];
+                    if (Smat) Smat[o] = ts * sc;
+                    if (Tmat) Tmat[o] = tt * sc;
+                    if (Vmat) Vmat[o] = tv * sc;
+                }
+            free(th);
+        }
+    shells_free(&S);
+    return 0;
+}
+
+/* Electron repulsion: contracted [e0|f0] over the primitive quartets, then HRR. */
+typedef struct {
+    const double *C; /* [ne * nf] contracted [e0|f0] */
+    int nf;
+    double AB[3], CD[3];
+} eri_hrr_t;
+
+static double eri_hrr_bra(const eri_hrr_t *h, const int a[3], const int b[3], int jf) {
+    int i = b[0] > 0 ? 0 : (b[1] > 0 ? 1 : (b[2] > 0 ? 2 : -1));
+    if (i < 0) return h->C[(size_t)tri_index(a[0], a[1], a[2]) * h->nf + jf];
+    int a1[3] = {a[0], a[1], a[2]}, b1[3] = {b[0], b[1], b[2]};
+    a1[i] += 1;
+    b1[i] -= 1;
+    return eri_hrr_bra(h, a1, b1, jf) + h->AB[i] * eri_hrr_bra(h, a, b1, jf);
+}
+
+static double eri_hrr(const eri_hrr_t *h, const int a[3], const int b[3], const int c[3], const int d[3]) {
+    int i = d[0] > 0 ? 0 : (d[1] > 0 ? 1 : (d[2] > 0 ? 2 : -1));
+    if (i < 0) return eri_hrr_bra(h, a, b, tri_index(c[0], c[1], c[2]));
+    int c1[3] = {c[0], c[1], c[2]}, d1[3] = {d[0], d[1], d[2]};
+    c1[i] += 1;
+    d1[i] -= 1;
+    return eri_hrr(h, a, b, c1, d1) + h->CD[i] * eri_hrr(h, a, b, c, d1);
+}
+
+/* out[nfa*nfb*nfc*nfd] = (ab|cd) of one shell quartet, component order a, b, c, d. */
+static void eri_quartet(const shell_t *A, const shell_t *B, const shell_t *Cs, const shell_t *D, double *out) {
+    const int Le = A->l + B->l, Lf = Cs->l + D->l, M = Le + Lf;
+    const int ne = tri_count(Le), nf = tri_count(Lf);
+    double *X = (double *)malloc(sizeof(double) * (size_t)(M + 1) * ne * nf);
+    double *Cc = (double *)calloc((size_t)ne * nf, sizeof(double));
+    double AB[3], CD[3], AB2 = 0.0, CD2 = 0.0, F[64];
+    for (int q = 0; q < 3; ++q) {
+        AB[q] = A->center[q] - B->center[q];
+        CD[q] = Cs->center[q] - D->center[q];
+        AB2 += AB[q] * AB[q];
+        CD2 += CD[q] * CD[q];
+    }
+#define XI(m, ie, jf) X[((size_t)(m) * ne + (ie)) * nf + (jf)]
+    for (int pa = 0; pa < A->n_prim; ++pa)
+        for (int pb = 0; pb < B->n_prim; ++pb) {
+            const double a = A->alpha[pa], b = B->alpha[pb], p = a + b;
+            double P[3], PA[3];
+            for (int q = 0; q < 3; ++q) {
+                P[q] = (a * A->center[q] + b * B->center[q]) / p;
+                PA[q] = P[q] - A->center[q];
+            }
+            const double Kab = exp(-a * b / p * AB2) * A->wnorm[pa] * B->wnorm[pb];
+            for (int pc = 0; pc < Cs->n_prim; ++pc)
+                for (int pd = 0; pd < D->n_prim; ++pd) {
+                    const double c = Cs->alpha[pc], d = D->alpha[pd], qq = c + d;
+                    double Q[3], QC[3], W[3], WP[3], WQ[3], PQ2 = 0.0;
+                    for (int k = 0; k < 3; ++k) {
+                        Q[k] = (c * Cs->center[k] + d * D->center[k]) / qq;
+                        QC[k] = Q[k] - Cs->center[k];
+                        W[k] = (p * P[k] + qq * Q[k]) / (p + qq);
+                        WP[k] = W[k] - P[k];
+                        WQ[k] = W[k] - Q[k];
+                        PQ2 += (P[k] - Q[k]) * (P[k] - Q[k]);
+                    }
+                    const double rho = p * qq / (p + qq);
+                    const double Kcd = exp(-c * d / qq * CD2) * Cs->wnorm[pc] * D->wnorm[pd];
+                    const double pre = 2.0 * pow(OR_PI, 2.5) / (p * qq * sqrt(p + qq)) * Kab * Kcd;
+                    or_boys(M, rho * PQ2, F);
+                    for (int m = 0; m <= M; ++m) XI(m, 0, 0) = pre * F[m];
+                    /* [e0|00]^(m) */
+                    for (int tot = 1; tot <= Le; ++tot)
+                        for (int x = tot; x >= 0; --x)
+                            for (int y = tot - x; y >= 0; --y) {
+                                int e[3] = {x, y, tot - x - y};
+                                int i = e[0] > 0 ? 0 : (e[1] > 0 ? 1 : 2);
+                                int e1[3] = {e[0], e[1], e[2]};
+                                e1[i] -= 1;
+                                const int ie = tri_index(e[0], e[1], e[2]), i1 = tri_index(e1[0], e1[1], e1[2]);
+                                int i2 = -1;
+                                if (e1[i] > 0) {
+                                    int e2[3] = {e1[0], e1[1], e1[2]};
+                                    e2[i] -= 1;
+                                    i2 = tri_index(e2[0], e2[1], e2[2]);
+                                }
+                                for (int m = 0; m <= M - tot; ++m) {
+                                    double v = PA[i] * XI(m, i1, 0) + WP[i] * XI(m + 1, i1, 0);
+                                    if (i2 >= 0) v += e1[i] / (2.0 * p) * (XI(m, i2, 0) - rho / p * XI(m + 1, i2, 0));
+                                    XI(m, ie, 0) = v;
+                                }
+                            }
+                    /* [e0|f0]^(m), f built up with e at every total */
+                    for (int ftot = 1; ftot <= Lf; ++ftot)
+                        for (int fx = ftot; fx >= 0; --fx)
+                            for (int fy = ftot - fx; fy >= 0; --fy) {
+                                int f[3] = {fx, fy, ftot - fx - fy};
+                                int i = f[0] > 0 ? 0 : (f[1] > 0 ? 1 : 2);
+                                int f1[3] = {f[0], f[1], f[2]};
+                                f1[i] -= 1;
+                                const int jf = tri_index(f[0], f[1], f[2]), j1 = tri_index(f1[0], f1[1], f1[2]);
+                                int j2 = -1;
+                                if (f1[i] > 0) {
+                                    int f2[3] = {f1[0], f1[1], f1[2]};
+                                    f2[i] -= 1;
+                                    j2 = tri_index(f2[0], f2[1], f2[2]);
+                                }
+                                for (int etot = 0; etot <= Le; ++etot)
+                                    for (int x = etot; x >= 0; --x)
+                                        for (int y = etot - x; y >= 0; --y) {
+                                            int e[3] = {x, y, etot - x - y};
+                                            const int ie = tri_index(e[0], e[1], e[2]);
+                                            int ie1 = -1;
+                                            if (e[i] > 0) {
+                                                int e1[3] = {e[0], e[1], e[2]};
+                                                e1[i] -= 1;
+                                                ie1 = tri_index(e1[0], e1[1], e1[2]);
+                                            }
+                                            for (int m = 0; m <= M - ftot - etot; ++m) {
+                                                double v = QC[i] * XI(m, ie, j1) + WQ[i] * XI(m + 1, ie, j1);
+                                                if (j2 >= 0)
+                                                    v += f1[i] / (2.0 * qq) * (XI(m, ie, j2) - rho / qq * XI(m + 1, ie, j2));
+                                                if (ie1 >= 0) v += e[i] / (2.0 * (p + qq)) * XI(m + 1, ie1, j1);
+                                                XI(m, ie, jf) = v;
+                                            }
+                                        }
+                            }
+                    for (int k = 0; k < ne * nf; ++k) Cc[k] += X[k];
+                }
+        }
+#undef XI
+    eri_hrr_t h;
+    h.C = Cc;
+    h.nf = nf;
+    for (int q = 0; q < 3; ++q) {
+        h.AB[q] = AB[q];
+        h.CD[q] = CD[q];
+    }
+    int o = 0;
+    for (int ia = 0; ia < A->nfn; ++ia)
+        for (int ib = 0; ib < B->nfn; ++ib)
+            for (int ic = 0; ic < Cs->nfn; ++ic)
+                for (int id = 0; id < D->nfn; ++id)
+                    out[o++] = eri_hrr(&h, comp_of(A->l, ia), comp_of(B->l, ib), comp_of(Cs->l, ic),
+                                       comp_of(D->l, id)) *
+                               A->scale[ia] * B->scale[ib] * Cs->scale[ic] * D->scale[id];
+    free(X);
+    free(Cc);
+}
+
+typedef struct {
+    const shells_t *S;
+    const int *quart; /* unique quartets (s1 >= s2, s3 >= s4, pair12 >= pair34), 4 ints each */
+    double *out;
+} eri_ctx_t;
+
+static void eri_worker(void *vctx, long b, long e, int tid) {
+    eri_ctx_t *X = (eri_ctx_t *)vctx;
+    const int n = X->S->n_ao;
+    double *buf = (double *)malloc(sizeof(double) * 10000);
+    for (long k = b; k < e; ++k) {
+        const int *q = X->quart + 4 * k;
+        const shell_t *A = &X->S->sh[q[0]], *B = &X->S->sh[q[1]], *C = &X->S->sh[q[2]], *D = &X->S->sh[q[3]];
+        eri_quartet(A, B, C, D, buf);
+        int o = 0;
+        for (int ia = 0; ia < A->nfn; ++ia)
+            for (int ib = 0; ib < B->nfn; ++ib)
+                for (int ic = 0; ic < C->nfn; ++ic)
+                    for (int id = 0; id < D->nfn; ++id) {
+                        const size_t i = A->ao_offset + ia, j = B->ao_offset + ib, kk = C->ao_offset + ic,
+                                     l = D->ao_offset + id;
+                        const double v = buf[o++];
+                        const size_t N = n;
+#define PUT(p, q, r, s) X->out[(((p) * N + (q)) * N + (r)) * N + (s)] = v
+                        PUT(i, j, kk, l); PUT(j, i, kk, l); PUT(i, j, l, kk); PUT(j, i, l, kk);
+                        PUT(kk, l, i, j); PUT(l, kk, i, j); PUT(kk, l, j, i); PUT(l, kk, j, i);
+#undef PUT
+                    }
+    }
+    free(buf);
+}
+
+/* Dense (ij|kl), out[n^4], from the unique shell quartets, all 8 permutations written. */
+int or_eri_full(const or_basis_t *b, int n_threads, double *out) {
+    shells_t S;
+    if (shells_build(b, &S) != 0) return -1;
+    const int ns = S.n_shells;
+    long nq = 0;
+    int *quart = (int *)malloc(sizeof(int) * 4 * (size_t)((long)ns * (ns + 1) / 2 * ((long)ns * (ns + 1) / 2 + 1) / 2 + 1));
+    for (int s1 = 0; s1 < ns; ++s1)
+        for (int s2 = 0; s2 <= s1; ++s2)
+            for (int s3 = 0; s3 <= s1; ++s3)
+                for (int s4 = 0; s4 <= (s3 == s1 ? s2 : s3); ++s4) {
+                    int *q = quart + 4 * nq++;
+                    q[0] = s1; q[1] = s2; q[2] = s3; q[3] = s4;
+                }
+    eri_ctx_t ctx = {&S, quart, out};
+    parallel_for(nq, n_threads, eri_worker, &ctx);
+    free(quart);
+    shells_free(&S);
+    return 0;
+}

@@ -97,6 +97,17 @@ int or_ao_dense(const or_basis_t *b, long n_pts, const double *pts, double *val,
 /* Analytic overlap S (Obara-Saika 1D recursion), [n_ao*n_ao]. */
 int or_overlap(const or_basis_t *b, double *S);
 
+/* Boys function F_m(T), m = 0..mmax (<= 63), long-double series / recursion. */
+void or_boys(int mmax, double T, double *F);
+/* Overlap S, kinetic T and nuclear attraction V = -sum_C Z_C (mu|1/r_C|nu)
+ * ([n_ao*n_ao] each, any may be NULL) over the n_atoms charges z at xyz (bohr):
+ * one_electron.rs:7-168 (Obara-Saika). */
+int or_one_electron(const or_basis_t *b, int n_atoms, const int *z, const double *xyz, double *S, double *T,
+                    double *V);
+/* Dense electron repulsion integrals (ij|kl), out[n_ao^4]: eri.rs:276-1018
+ * (Obara-Saika VRR, Head-Gordon-Pople HRR). */
+int or_eri_full(const or_basis_t *b, int n_threads, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,9 @@ def lib():
                                      C.c_int, _dp]
         L.or_rho_points.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, C.c_int, _dp]
         L.or_ao_dense.argtypes = [C.POINTER(_Basis), C.c_long, _dp, _dp, _dp]
+        L.or_boys.argtypes = [C.c_int, C.c_double, _dp]
+        L.or_one_electron.argtypes = [C.POINTER(_Basis), C.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.or_eri_full.argtypes = [C.POINTER(_Basis), C.c_int, _dp]
         _lib = L
     return _lib
 
@@ -228,3 +231,31 @@ def ao_dense(ob: OracleBasis, grid_rows):
     if rc != 0:
         raise RuntimeError(rc)
     return val.reshape(n, ob.n_ao), grad.reshape(n, ob.n_ao, 3)
+
+
+def boys(mmax, T):
+    out = np.zeros(mmax + 1)
+    lib().or_boys(mmax, float(T), out)
+    return out
+
+
+def one_electron(ob: OracleBasis, mol):
+    """(S, T, V) of the basis on molecule mol (V over its nuclei)."""
+    z = np.ascontiguousarray(mol.z_array(), dtype=np.int32)
+    xyz = np.ascontiguousarray(mol.coords_bohr().ravel())
+    n = ob.n_ao
+    S, T, V = np.zeros(n * n), np.zeros(n * n), np.zeros(n * n)
+    rc = lib().or_one_electron(C.byref(ob.struct), len(z), z, xyz, S, T, V)
+    if rc != 0:
+        raise RuntimeError(rc)
+    return S.reshape(n, n), T.reshape(n, n), V.reshape(n, n)
+
+
+def eri_full(ob: OracleBasis, threads=None):
+    n = ob.n_ao
+    out = np.zeros(n ** 4)
+    nt = threads or min(16, os.cpu_count() or 1)
+    rc = lib().or_eri_full(C.byref(ob.struct), nt, out)
+    if rc != 0:
+        raise RuntimeError(rc)
+    return out.reshape(n, n, n, n)

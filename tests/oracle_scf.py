@@ -1,0 +1,82 @@
+"""Plain FP64 SCF on the oracle's integrals and Vxc (test infrastructure).
+
+A CPU restatement of the closed-shell SCF energy of
+crates/calchem-scf/src/rks.rs:14-44,253-280 (F = h + J - (c_x / 2) K + V_xc,
+E = sum P (h + G/2) + E_xc + E_nuc) with Pulay DIIS on the commutator
+FPS - SPF (rks.rs:282-291, diis.rs:183-303), converged tightly.  Dense
+(ij|kl) from the oracle: only for small molecules.  The converged energy is a
+property of the Hamiltonian, so it checks the product's SCF whatever path
+that takes to convergence.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_lib as O
+
+EXACT_EXCHANGE = {None: 1.0, "svwn5": 0.0, "blyp": 0.0, "pbe": 0.0, "b3lyp": 0.2, "b3lyp5": 0.2, "pbe0": 0.25}
+
+
+def nuclear_repulsion(mol) -> float:
+    R = mol.coords_bohr()
+    z = mol.z_array().astype(np.float64)
+    e = 0.0
+    for i in range(len(z)):
+        for j in range(i):
+            e += z[i] * z[j] / np.linalg.norm(R[i] - R[j])
+    return e
+
+
+def scf(mol, basis, func=None, grid_rows=None, P0=None, conv_e=1e-11, conv_d=1e-10, max_iter=200):
+    """Closed-shell SCF: func=None is RHF; otherwise RKS on the grid rows
+    (x, y, z, w, ...).  Returns (E_total, P, iterations)."""
+    ob = O.OracleBasis(basis)
+    S, T, V = O.one_electron(ob, mol)
+    H = T + V
+    eri = O.eri_full(ob)
+    cx = EXACT_EXCHANGE[func]
+    n_occ = mol.n_electrons() // 2
+    w, U = np.linalg.eigh(S)
+    X = U @ np.diag(w ** -0.5) @ U.T
+    e_nuc = nuclear_repulsion(mol)
+
+    def density(F):
+        _, Cp = np.linalg.eigh(X.T @ F @ X)
+        C = X @ Cp
+        return 2.0 * C[:, :n_occ] @ C[:, :n_occ].T
+
+    P = density(H) if P0 is None else P0.copy()
+    focks, errs = [], []
+    e_old = None
+    for it in range(1, max_iter + 1):
+        J = np.einsum("ijkl,kl->ij", eri, P)
+        K = np.einsum("ikjl,kl->ij", eri, P)
+        G = J - 0.5 * cx * K
+        F = H + G
+        e = float(np.sum(P * (H + 0.5 * G))) + e_nuc
+        if func is not None:
+            exc, Vxc, _ = O.vxc_rks(ob, grid_rows, func, P)
+            F = F + Vxc
+            e += exc
+        err = F @ P @ S - S @ P @ F
+        focks.append(F)
+        errs.append(err)
+        del focks[:-8], errs[:-8]
+        Fx = F
+        if len(focks) > 1:
+            m = len(focks)
+            B = -np.ones((m + 1, m + 1))
+            B[m, m] = 0.0
+            for i in range(m):
+                for j in range(m):
+                    B[i, j] = np.sum(errs[i] * errs[j])
+            rhs = np.zeros(m + 1)
+            rhs[m] = -1.0
+            c = np.linalg.lstsq(B, rhs, rcond=None)[0]
+            Fx = sum(c[i] * focks[i] for i in range(m))
+        Pn = density(Fx)
+        if e_old is not None and abs(e - e_old) < conv_e and np.abs(Pn - P).max() < conv_d \
+                and np.linalg.norm(err) < 1e-7:
+            return e, P, it
+        P, e_old = Pn, e
+    raise RuntimeError("oracle SCF did not converge")
