@@ -60,6 +60,8 @@ EXPORTS = [
     "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
     "calchem_hip_grid_build", "calchem_hip_xc_eval", "calchem_hip_xc_eval_device",
     "calchem_hip_xc_eval_device_noalloc",
+    "calchem_hip_int_create", "calchem_hip_int_release", "calchem_hip_int_one_electron", "calchem_hip_int_eri_diag",
+    "calchem_hip_int_eri_columns", "calchem_hip_int_cholesky", "calchem_hip_int_jk",
 ]
 
 _lib = None
@@ -105,6 +107,13 @@ def load() -> C.CDLL:
     L.calchem_hip_xc_eval.argtypes = [i32, C.c_longlong, vp, vp, vp, vp]
     L.calchem_hip_xc_eval_device.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
     L.calchem_hip_xc_eval_device_noalloc.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
+    L.calchem_hip_int_create.argtypes = [vp, i32, vp, i32, vp, i32, i32, i32, vp, u64]
+    L.calchem_hip_int_release.argtypes = [u64]
+    L.calchem_hip_int_one_electron.argtypes = [u64, vp, vp, vp]
+    L.calchem_hip_int_eri_diag.argtypes = [u64, vp]
+    L.calchem_hip_int_eri_columns.argtypes = [u64, i32, vp, vp, vp]
+    L.calchem_hip_int_cholesky.argtypes = [u64, C.c_double, i32, C.POINTER(C.c_int)]
+    L.calchem_hip_int_jk.argtypes = [u64, i32, vp, i32, vp, vp, vp]
     _lib = L
     return L
 

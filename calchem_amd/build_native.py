@@ -2,10 +2,11 @@
 the MFMA throughput probe calchem_amd/libcalchem_probe.so used by bench.py.
 
 Every call compiles from source (no incremental reuse), so the library always
-matches the tree.  Also used by __graft_entry__.build().
+matches the tree; the objects compile side by side (MAX_JOBS caps the workers).  Also used by __graft_entry__.build().
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import shutil
 import subprocess
@@ -17,7 +18,9 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcalchem_hip.so")
 PROBE_LIB = os.path.join(HERE, "libcalchem_probe.so")
 PROBE_SOURCE = "probe_mfma.hip"
-SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip", "xc_points.hip"]
+SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip", "xc_points.hip", "integrals.hip"]
+ERI_SOURCE = "eri_classes.hip"
+ERI_BRA_CLASSES = 6
 ARCH = os.environ.get("CALCHEM_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
@@ -37,23 +40,38 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the calchem_amd HIP library cannot be built")
 
 
+def _compile_jobs(tmp: str):
+    """(source, extra flags, object) for every object of the library."""
+    jobs = [(src, [], os.path.join(tmp, src.replace(".hip", ".o"))) for src in SOURCES]
+    # the repulsion kernels: one object per bra angular-momentum class
+    for k in range(ERI_BRA_CLASSES):
+        jobs.append((ERI_SOURCE, [f"-DERI_BRA_CLASS={k}"], os.path.join(tmp, f"eri_classes_{k}.o")))
+    return jobs
+
+
 def build(verbose: bool = False) -> str:
     cc = hipcc()
     tmp = tempfile.mkdtemp(prefix="calchem_hip_build_")
     try:
-        objs = []
-        for src in SOURCES:
-            obj = os.path.join(tmp, src.replace(".hip", ".o"))
-            cmd = [cc, *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
-            r = subprocess.run(cmd, capture_output=True, text=True)
+        jobs = _compile_jobs(tmp)
+
+        def run(job):
+            src, extra, obj = job
+            cmd = [cc, *HIPCC_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj]
+            return job, subprocess.run(cmd, capture_output=True, text=True)
+
+        workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", "0")) or (os.cpu_count() or 4)))
+        with concurrent.futures.ThreadPoolExecutor(max_workers=workers) as pool:
+            results = list(pool.map(run, jobs))
+        for (src, extra, obj), r in results:
             if r.returncode != 0:
                 sys.stderr.write(r.stdout + r.stderr)
-                raise RuntimeError(f"hipcc failed on {src}")
+                raise RuntimeError(f"hipcc failed on {src} {' '.join(extra)}")
             if verbose and r.stderr.strip():
                 sys.stderr.write(r.stderr)
-            objs.append(obj)
+        objs = [obj for (_, _, obj), _ in results]
         out_tmp = LIB + ".tmp"
-        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out_tmp, *objs]
+        cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", out_tmp, *objs, "-lrocblas"]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)

@@ -230,6 +230,37 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats *out);
 long calchem_hip_grid_build(int n_atoms, const int *z, const double *xyz_bohr, int level, int n_rad, int n_ang,
                             int radial_min, double *out, long max_out);
 
+/* ---- Integrals of the SCF Hamiltonian ----
+ * The SCF row beside the Vxc build (crates/calchem-scf/src/rks.rs:131-615):
+ * one-electron integrals (compute_overlap / compute_kinetic / compute_nuclear,
+ * crates/calchem-int/src/one_electron.rs:115-168) and J / K
+ * (RksFockBuilder, crates/calchem-scf/src/fock_builder.rs; exact integrals in
+ * the reference, direct_fock.rs:1028) from Cholesky-decomposed repulsion
+ * integrals resident on the device.  Shells up to d (-7 above).
+ * create: the basis in the Vxc records above (same shell / primitive /
+ * component conventions) and the nuclei atoms_xyzz[n_atoms*4] = (x, y, z, Z)
+ * in bohr; it also computes the repulsion diagonal.  A handle in use: -2. */
+int calchem_hip_int_create(const CalchemAoShell *shells, int n_shells, const CalchemAoPrim *prims, int n_prims,
+                           const CalchemAoComp *comps, int n_comps, int n_ao, int n_atoms, const double *atoms_xyzz,
+                           uint64_t handle);
+int calchem_hip_int_release(uint64_t handle);
+/* S, T and V = -sum_C Z_C (mu|1/r_C|nu), each n_ao^2 row-major (host memory). */
+int calchem_hip_int_one_electron(uint64_t handle, double *S, double *T, double *V);
+/* (mu nu|mu nu) for every packed pair mu >= nu, index mu (mu + 1) / 2 + nu (host, n_ao (n_ao + 1) / 2). */
+int calchem_hip_int_eri_diag(uint64_t handle, double *out);
+/* Exact repulsion columns: for each shell pair (shell_a[k], shell_b[k]) its
+ * n_a n_b columns (component of shell_a major), each holding
+ * (mu nu|lam sig) over the packed pairs mu >= nu (host memory). */
+int calchem_hip_int_eri_columns(uint64_t handle, int n_kets, const int *shell_a, const int *shell_b, double *out);
+/* Pivoted Cholesky decomposition of the repulsion integrals down to residual
+ * diagonal tau (max_rank 0: up to 25 n_ao vectors); *rank_out = vectors kept. */
+int calchem_hip_int_cholesky(uint64_t handle, double tau, int max_rank, int *rank_out);
+/* J[mu nu] = sum (mu nu|lam sig) P[lam sig] and K[mu nu] = sum (mu lam|nu sig) P[lam sig]
+ * (n_ao^2 each, either may be NULL) for P = Cpos Cpos^T - Cneg Cneg^T, with
+ * Cpos (n_ao x n_pos) and Cneg (n_ao x n_neg) row-major in host memory. */
+int calchem_hip_int_jk(uint64_t handle, int n_pos, const double *Cpos, int n_neg, const double *Cneg, double *J,
+                       double *K);
+
 /* ---- The functional on caller points ----
  * No reference entry point: evaluates eval_xc_spin (crates/calchem-dft/src/xc.rs:159-223)
  * with the device code the builds use, at arbitrary spin invariants, so that
