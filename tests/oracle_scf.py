@@ -80,3 +80,20 @@ def scf(mol, basis, func=None, grid_rows=None, P0=None, conv_e=1e-11, conv_d=1e-
             return e, P, it
         P, e_old = Pn, e
     raise RuntimeError("oracle SCF did not converge")
+
+
+class OracleIntegrals:
+    """The oracle's integrals behind the interface of calchem_amd.integrals.IntegralEngine
+    that the SAD guess uses (one_electron, eri_full, close)."""
+
+    def __init__(self, mol, basis):
+        self.mol, self.ob = mol, O.OracleBasis(basis)
+
+    def one_electron(self):
+        return O.one_electron(self.ob, self.mol)
+
+    def eri_full(self, basis):
+        return O.eri_full(self.ob)
+
+    def close(self):
+        pass
