@@ -77,23 +77,20 @@ hipError_t launch_one(int cls, const OneArgs& a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 // Small kernels of the decomposition and of J / K.
 // ---------------------------------------------------------------------------
-// dst[i] = src[i] * s over n elements (a pivot column scaled into a vector).
-__global__ void k_scale_copy(const double* __restrict__ src, double* __restrict__ dst, double s, long long n) {
+// out[i + t * n] = G[i + cols[t] * n]: the pivot columns of a block.
+__global__ void k_gather_cols(const double* __restrict__ G, long long n, const int* __restrict__ cols, int m,
+                              double* __restrict__ out) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = src[i] * s;
+    if (i >= n * m) return;
+    out[i] = G[(i % n) + (long long)cols[i / n] * n];
 }
-// G[:, k] -= v * c[k] for the ncol columns of G (leading dimension ld).
-__global__ void k_rank1_cols(double* __restrict__ G, long long ld, int ncol, const double* __restrict__ v,
-                             const double* __restrict__ c, long long n) {
+// D[i] -= sum_t V[i + t n]^2 over the m new vectors.
+__global__ void k_diag_update_block(double* __restrict__ D, const double* __restrict__ V, long long n, int m) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double vi = v[i];
-    for (int k = 0; k < ncol; ++k) G[(long long)k * ld + i] -= vi * c[k];
-}
-// D[i] -= v[i]^2.
-__global__ void k_diag_update(double* __restrict__ D, const double* __restrict__ v, long long n) {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) D[i] -= v[i] * v[i];
+    double s = 0.0;
+    for (int t = 0; t < m; ++t) s += V[i + (long long)t * n] * V[i + (long long)t * n];
+    D[i] -= s;
 }
 // out[j + k * ldo] = src[rows[j] + k * lds] for j < nrow, k < ncol (row gather).
 __global__ void k_gather_rows(const double* __restrict__ src, long long lds, const int* __restrict__ rows, int nrow,
@@ -171,7 +168,7 @@ struct IntPlan {
     int rank = 0;
     double tau = 0.0;
     Buf Lf;       // [rank][n][n]
-    Buf work_y, work_p, work_d, work_c;
+    Buf work_y, work_p, work_d, work_c, work_k;
     rocblas_handle blas = nullptr;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -308,7 +305,7 @@ int build_cholesky(IntPlan& P, double tau, int max_rank, int* rank_out) {
         hipStreamSynchronize(P.stream) != hipSuccess)
         return kLaunchFail;
     constexpr int kMaxCols = 512;
-    if (!G.alloc(sizeof(double) * (size_t)np * kMaxCols) || !coef.alloc(sizeof(double) * kMaxCols) ||
+    if (!G.alloc(sizeof(double) * (size_t)np * kMaxCols) || !coef.alloc(sizeof(double) * kMaxCols * kMaxCols) ||
         !rows.alloc(sizeof(int) * kMaxCols) || !sub.alloc(sizeof(double) * (size_t)kMaxCols * std::max(cap, kMaxCols)))
         return kAllocFail;
     double* Lpd = Lp.as<double>();
@@ -378,36 +375,62 @@ int build_cholesky(IntPlan& P, double tau, int max_rank, int* rank_out) {
                 hipStreamSynchronize(P.stream) != hipSuccess)
                 return kLaunchFail;
         }
-        // Gs[j * ncol + i] = residual (fp_i | col j)
+        // Gs[j * ncol + i] = residual (fp_i | col j).  Pivoted Cholesky of this
+        // block on the host: pivots p_t and the factor's rows at the pivots,
+        // Ls[t][s] = l_s[p_t]; the block's vectors are then V = G[:, p] Ls^{-T}.
+        std::vector<int> piv;
+        std::vector<double> lcols;  // ncol x m, column t = l_t
         std::vector<char> used(ncol, 0);
-        std::vector<double> cvec(ncol);
-        while (r < cap) {
+        std::vector<double> dres(ncol);  // residual diagonal of the block
+        for (int j = 0; j < ncol; ++j) dres[j] = Gs[(size_t)j * ncol + j];
+        while (r + (int)piv.size() < cap) {
             int jb = -1;
             double best = -1.0;
             for (int j = 0; j < ncol; ++j)
-                if (!used[j] && cfp[j] >= 0 && Gs[(size_t)j * ncol + j] > best) {
-                    best = Gs[(size_t)j * ncol + j];
+                if (!used[j] && cfp[j] >= 0 && dres[j] > best) {
+                    best = dres[j];
                     jb = j;
                 }
             if (jb < 0 || best < thr) break;
             const double inv = 1.0 / std::sqrt(best);
-            double* v = Lpd + (size_t)r * np;
-            hipLaunchKernelGGL(k_scale_copy, dim3(nblk(np)), dim3(256), 0, P.stream, Gd + (size_t)jb * np, v, inv, np);
-            for (int k = 0; k < ncol; ++k) cvec[k] = (used[k] || k == jb) ? 0.0 : Gs[(size_t)k * ncol + jb] * inv;
-            if (hipMemcpyAsync(coef.p, cvec.data(), sizeof(double) * ncol, hipMemcpyHostToDevice, P.stream) != hipSuccess)
-                return kLaunchFail;
-            hipLaunchKernelGGL(k_rank1_cols, dim3(nblk(np)), dim3(256), 0, P.stream, Gd, np, ncol, v,
-                               coef.as<double>(), np);
-            hipLaunchKernelGGL(k_diag_update, dim3(nblk(np)), dim3(256), 0, P.stream, Dd.as<double>(), v, np);
-            // host copy of the block: the same rank-one update
-            std::vector<double> vc(ncol);
-            for (int k = 0; k < ncol; ++k) vc[k] = Gs[(size_t)k * ncol + jb] * inv;
-            for (int k = 0; k < ncol; ++k)
-                for (int i = 0; i < ncol; ++i) Gs[(size_t)k * ncol + i] -= vc[i] * vc[k];
+            const size_t t = piv.size();
+            lcols.resize((t + 1) * ncol);
+            double* lt = lcols.data() + t * ncol;
+            for (int i = 0; i < ncol; ++i) {
+                double v = Gs[(size_t)jb * ncol + i];
+                for (size_t s2 = 0; s2 < t; ++s2) v -= lcols[s2 * ncol + i] * lcols[s2 * ncol + jb];
+                lt[i] = v * inv;
+            }
+            for (int i = 0; i < ncol; ++i) dres[i] -= lt[i] * lt[i];
             used[jb] = 1;
-            ++r;
-            // the copy of cvec must not be overwritten before the update has read it
+            piv.push_back(jb);
+        }
+        const int m = (int)piv.size();
+        if (m == 0) break;  // no candidate above the threshold: the residual is below it to rounding
+        {
+            std::vector<double> Ls((size_t)m * m, 0.0);  // column-major, lower
+            std::vector<int> pcol(m);
+            for (int t = 0; t < m; ++t) {
+                pcol[t] = piv[t];
+                for (int s2 = 0; s2 <= t; ++s2) Ls[t + (size_t)s2 * m] = lcols[(size_t)s2 * ncol + piv[t]];
+            }
+            double* Lsd = coef.as<double>();
+            int* pcd = rows.as<int>();
+            if (hipMemcpyAsync(Lsd, Ls.data(), sizeof(double) * m * m, hipMemcpyHostToDevice, P.stream) != hipSuccess ||
+                hipMemcpyAsync(pcd, pcol.data(), sizeof(int) * m, hipMemcpyHostToDevice, P.stream) != hipSuccess)
+                return kLaunchFail;
+            double* V = Lpd + (size_t)r * np;
+            hipLaunchKernelGGL(k_gather_cols, dim3(nblk(np * m)), dim3(256), 0, P.stream, Gd, np, pcd, m, V);
+            const double one = 1.0;
+            if (rocblas_dtrsm(P.blas, rocblas_side_right, rocblas_fill_lower, rocblas_operation_transpose,
+                              rocblas_diagonal_non_unit, (rocblas_int)np, m, &one, Lsd, m, V, (rocblas_int)np) !=
+                rocblas_status_success)
+                return kLaunchFail;
+            hipLaunchKernelGGL(k_diag_update_block, dim3(nblk(np)), dim3(256), 0, P.stream, Dd.as<double>(), V, np, m);
+            // the host buffers above must outlive the copies
             if (hipStreamSynchronize(P.stream) != hipSuccess) return kLaunchFail;
+            r += m;
+            for (int t = 0; t < m; ++t) used[piv[t]] = 1;
         }
         if (hipMemcpyAsync(Dh.data(), Dd.p, sizeof(double) * np, hipMemcpyDeviceToHost, P.stream) != hipSuccess ||
             hipStreamSynchronize(P.stream) != hipSuccess)
@@ -430,28 +453,54 @@ int build_cholesky(IntPlan& P, double tau, int max_rank, int* rank_out) {
 }
 
 // K += sign * sum_Q (L^Q C)(L^Q C)^T for C (n x k, row-major on the device:
-// column-major k x n).  Q in blocks so that the GEMM output stays below ~2 GiB.
+// column-major k x n).  Y^Q = L^Q C for a block of Q by one GEMM; the sum over
+// (Q, i) of Y Y^T has a tiny n x n output and a long reduction, so it runs as
+// a strided-batched GEMM over chunks of Q (enough workgroups to fill the
+// device) whose partial matrices are then added in chunk order.
+__global__ void k_sum_partials(const double* __restrict__ part, int n_part, long long nn, double sign,
+                               double* __restrict__ K, int accumulate) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nn) return;
+    double s = 0.0;
+    for (int b = 0; b < n_part; ++b) s += part[(long long)b * nn + i];
+    K[i] = accumulate ? K[i] + sign * s : sign * s;
+}
+
 int add_exchange(IntPlan& P, const double* dC, int k, double sign, double* dK, bool first) {
     const int n = P.n_ao, r = P.rank;
-    const long long per_q = (long long)n * k;
+    const long long nn = (long long)n * n, per_q = (long long)n * k;
     const long long qb = std::max<long long>(1, std::min<long long>(r, (256LL << 20) / std::max<long long>(per_q, 1)));
-    if (!P.work_y.ensure(sizeof(double) * (size_t)(per_q * qb))) return kAllocFail;
+    // Q per batch entry: about 4096 columns of Z each
+    const int cq = (int)std::max<long long>(1, std::min<long long>(qb, 4096 / std::max(k, 1)));
+    const int max_parts = (int)((qb + cq - 1) / cq);
+    if (!P.work_y.ensure(sizeof(double) * (size_t)(per_q * qb)) ||
+        !P.work_k.ensure(sizeof(double) * (size_t)(nn * max_parts)))
+        return kAllocFail;
     double* Y = P.work_y.as<double>();
-    bool beta0 = first;
+    double* part = P.work_k.as<double>();
+    bool acc = !first;
     for (long long q0 = 0; q0 < r; q0 += qb) {
         const int nq = (int)std::min<long long>(qb, r - q0);
         const double one = 1.0, zero = 0.0;
         // Y (n nq x k, column-major) = M1^T C with M1 = L block viewed as n x (n nq)
         if (rocblas_dgemm(P.blas, rocblas_operation_transpose, rocblas_operation_transpose, n * nq, k, n, &one,
-                          P.Lf.as<double>() + q0 * (long long)n * n, n, dC, k, &zero, Y, n * nq) !=
-            rocblas_status_success)
+                          P.Lf.as<double>() + q0 * nn, n, dC, k, &zero, Y, n * nq) != rocblas_status_success)
             return kLaunchFail;
-        // K += sign Z Z^T, Z = Y viewed as n x (nq k)
-        const double beta = beta0 ? 0.0 : 1.0;
-        if (rocblas_dsyrk(P.blas, rocblas_fill_lower, rocblas_operation_none, n, nq * k, &sign, Y, n, &beta, dK, n) !=
-            rocblas_status_success)
+        // Z = Y viewed as n x (nq k); chunk b = columns [b cq k, (b + 1) cq k), the last one shorter
+        const int full = nq / cq, rest = nq - full * cq;
+        if (full > 0 &&
+            rocblas_dgemm_strided_batched(P.blas, rocblas_operation_none, rocblas_operation_transpose, n, n, cq * k,
+                                          &one, Y, n, (rocblas_stride)cq * k * n, Y, n, (rocblas_stride)cq * k * n,
+                                          &zero, part, n, (rocblas_stride)nn, full) != rocblas_status_success)
             return kLaunchFail;
-        beta0 = false;
+        if (rest > 0 &&
+            rocblas_dgemm(P.blas, rocblas_operation_none, rocblas_operation_transpose, n, n, rest * k, &one,
+                          Y + (long long)full * cq * k * n, n, Y + (long long)full * cq * k * n, n, &zero,
+                          part + (long long)full * nn, n) != rocblas_status_success)
+            return kLaunchFail;
+        hipLaunchKernelGGL(k_sum_partials, dim3(nblk(nn)), dim3(256), 0, P.stream, part, full + (rest > 0 ? 1 : 0),
+                           nn, sign, dK, acc ? 1 : 0);
+        acc = true;
     }
     return kOk;
 }
@@ -779,7 +828,7 @@ int calchem_hip_int_jk(uint64_t handle, int n_pos, const double* Cpos, int n_neg
         if (rc != kOk) return rc;
         if (first)
             hipLaunchKernelGGL(k_zero, dim3(nblk((long long)nn)), dim3(256), 0, P.stream, dK, (long long)nn);
-        else
+        else  // exact symmetry (the GEMM's two triangles may differ in the last bit)
             hipLaunchKernelGGL(k_symmetrize_lower, dim3(nblk((long long)nn)), dim3(256), 0, P.stream, dK, n);
         if (hipMemcpyAsync(K, dK, sizeof(double) * nn, hipMemcpyDeviceToHost, P.stream) != hipSuccess) return kLaunchFail;
     }
