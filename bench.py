@@ -35,6 +35,46 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def scf_leg(mol, basis, grid, func, learned_clusters: int = 4):
+    """FP64 SCF against the mixed-then-FP64 SCF (gate |dE| < 1e-8 Eh); with a
+    trained learned-guess model, SAD against learned starts on a few held-out
+    16-water clusters (iteration counts)."""
+    from calchem_amd.scf import RksSolver, ScfOptions
+    out = {}
+    t0 = time.perf_counter()
+    s = RksSolver(mol, basis, func, grid, ScfOptions())
+    out["setup_s"] = {k: round(v, 4) for k, v in s.setup_s.items()}
+    out["cholesky_rank"] = s.rank
+    for prec in ("fp64", "mixed"):
+        s.opts.precision = prec
+        t1 = time.perf_counter()
+        r = s.run()
+        out[prec] = {"energy": r.total_energy, "iterations": r.iterations, "seconds": time.perf_counter() - t1,
+                     "mixed_builds": r.precision_history.count("fp32")}
+    s.close()
+    out["dE_mixed_minus_fp64"] = out["mixed"]["energy"] - out["fp64"]["energy"]
+    out["gate_1e-8"] = abs(out["dE_mixed_minus_fp64"]) < 1e-8
+    log(f"SCF {func}: FP64 {out['fp64']['energy']:.10f} ({out['fp64']['iterations']} it, "
+        f"{out['fp64']['seconds']:.2f}s), mixed-then-FP64 {out['mixed']['energy']:.10f} "
+        f"({out['mixed']['iterations']} it, {out['mixed']['mixed_builds']} mixed builds), "
+        f"dE {out['dE_mixed_minus_fp64']:.2e}, setup {out['setup_s']}, total {time.perf_counter() - t0:.1f}s")
+    if not out["gate_1e-8"]:
+        log("mixed-then-FP64 SCF energy differs from the FP64 SCF by 1e-8 Eh or more: no result line")
+        return None
+    from calchem_amd import learned_guess as LG
+    if os.path.exists(LG.DEFAULT_MODEL) and learned_clusters > 0:
+        model = LG.LearnedGuess.load(LG.DEFAULT_MODEL)
+        rows = LG._eval_chunk((LG.DEFAULT_MODEL, [(k, 16, 100000 * 1004 + k) for k in range(learned_clusters)]))
+        out["learned_guess"] = {
+            "clusters": learned_clusters, "waters": 16,
+            "mean_iters_sad": float(np.mean([r["iters_sad"] for r in rows])),
+            "mean_iters_learned": float(np.mean([r["iters_learned_purified"] for r in rows])),
+            "mean_iters_learned_unpurified": float(np.mean([r["iters_learned"] for r in rows])),
+            "max_abs_dE": float(max(abs(r["dE"]) for r in rows)), "trained_on": model.meta.get("n_train_clusters")}
+        log(f"learned guess on {learned_clusters} held-out 16-water clusters: {out['learned_guess']}")
+    return out
+
+
 def timed_builds(eng, func, prec, dP, dV, den, stream, steps, warmup, hr):
     for _ in range(warmup):
         eng.build_device(func, dP.address, dV.address, den.address, stream.address, prec)
@@ -158,7 +198,7 @@ def cpu_baseline(basis, grid, func, P, target_s, threads):
 
 def bench_batched(args) -> int:
     """Many-molecule workload: clusters of 2-32 waters (sizes seeded 3), PBE /
-    6-31G(d) on a 50x194 grid per atom.  One batched plan (one launch per stage
+    6-31G(d,p) (50-800 AO) on a 50x194 grid per atom.  One batched plan (one launch per stage
     over all molecules), the same build replayed from a captured HIP graph,
     and a sequential loop over per-molecule plans on the same stream; every
     molecule's E_xc must agree bitwise between batched and sequential."""
@@ -172,7 +212,7 @@ def bench_batched(args) -> int:
     mols = configs.batching_clusters(args.clusters, 3)
     systems = []
     for k, mol in enumerate(mols):
-        basis = BasisSet.from_name("6-31g(d)", mol)
+        basis = BasisSet.from_name("6-31g(d,p)", mol)
         grid = MolecularGrid.explicit(mol, 50, 194)
         P = configs.random_density(basis, 5 * (mol.n_atoms // 3), 5000 + k)
         systems.append((basis, grid, P))
@@ -269,6 +309,7 @@ def main() -> int:
     ap.add_argument("--func", default=None)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-scf", action="store_true", help="skip the SCF leg")
     ap.add_argument("--block-points", type=int, default=0)
     ap.add_argument("--group-points", type=int, default=0)
     ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16])
@@ -332,10 +373,26 @@ def main() -> int:
         en = den.to_numpy((2,))
         ms, wall = timed_builds(eng, func, prec, dP, dV, den, stream, args.steps, args.warmup, hr)
         st, kms = kernel_profile(eng, func, prec, dP, dV, den, stream, max(3, min(args.steps, 10)))
+        # the same build replayed from a captured HIP graph (launch overhead of small molecules)
+        g = eng.graph_create(func, dP.address, dV.address, den.address, prec)
+        for _ in range(args.warmup):
+            eng.graph_launch(g, stream.address)
+        stream.synchronize()
+        ev0, ev1 = hr.Event(), hr.Event()
+        ev0.record(stream)
+        for _ in range(args.steps):
+            eng.graph_launch(g, stream.address)
+        ev1.record(stream)
+        ev1.synchronize()
+        graph_ms = ev0.elapsed_ms(ev1) / args.steps
+        eng.graph_destroy(g)
+        if float(den.to_numpy((2,))[0]) != float(en[0]):
+            log(f"{prec}: graph replay E_xc differs from the eager build: no result line")
+            return 6
         results[prec] = {"V": V, "exc": float(en[0]), "nelec": float(en[1]), "ms": ms, "wall": wall,
-                         "kernel_ms": kms, "stats": st}
-        log(f"{prec}: {ms:.3f} ms/build ({1000.0 / ms:.2f} builds/s), E_xc={en[0]:.12f} N={en[1]:.10f}, "
-            f"kernels {kms}")
+                         "kernel_ms": kms, "stats": st, "graph_ms": graph_ms}
+        log(f"{prec}: {ms:.3f} ms/build ({1000.0 / ms:.2f} builds/s), graph replay {graph_ms:.3f} ms, "
+            f"E_xc={en[0]:.12f} N={en[1]:.10f}, kernels {kms}")
 
     # XC nuclear gradient of the same workload (restricted, FP64), host buffers.
     eng.set_atoms(mol, grid)
@@ -369,6 +426,14 @@ def main() -> int:
         uks_ms[prec] = ev0.elapsed_ms(ev1) / args.steps
     uks.close()
     log(f"UKS builds: {uks_ms['fp64']:.3f} ms FP64, {uks_ms['fp32']:.3f} ms mixed")
+
+    # The SCF around the build: B3LYP on the same molecule, grid and basis, all-FP64
+    # against mixed Vxc builds until the DIIS error falls below 1e-5 (then FP64).
+    scf = None
+    if not args.no_scf:
+        scf = scf_leg(mol, basis, grid, func)
+        if scf is None:
+            return 5
 
     # Host-buffer path (P copied in, V copied out over PCIe, synchronous), for reference only.
     pcie = {}
@@ -453,11 +518,13 @@ def main() -> int:
         "fp64_builds_per_s": 1000.0 / r64["ms"],
         "fp32_builds_per_s": 1000.0 / r32["ms"],
         "mixed_vs_fp64_speedup": r64["ms"] / r32["ms"],
+        "graph_replay_ms_per_build": {"fp64": r64["graph_ms"], "fp32": r32["graph_ms"]},
         "pcie_inclusive_builds_per_s": pcie,
         "xc_gradient_ms": grad_ms,
         "uks_ms_per_build": uks_ms,
         "mixed_errors_vs_fp64_gpu": {"dE_xc": dE, "max_dV": dV32, "dN": dN, "gate_1e-7": gate},
         "fp64_gpu_vs_oracle_sample": parity,
+        "scf": scf,
         "build_tflops": build_tflops,
         "kernel_ms": kms,
         "roofline": {"bound": "mfma", "kernel": kname, "launches": "all launches of one build",

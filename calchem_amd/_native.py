@@ -58,10 +58,13 @@ EXPORTS = [
     "calchem_hip_vxc_graph_create", "calchem_hip_vxc_graph_launch", "calchem_hip_vxc_graph_destroy",
     "calchem_hip_vxc_set_atoms", "calchem_hip_vxc_xc_gradient",
     "calchem_hip_vxc_set_profiling", "calchem_hip_vxc_stats",
-    "calchem_hip_grid_build", "calchem_hip_xc_eval", "calchem_hip_xc_eval_device",
+    "calchem_hip_grid_build", "calchem_hip_grid_streaming_tasks", "calchem_hip_grid_task_points",
+    "calchem_hip_xc_eval", "calchem_hip_xc_eval_device",
     "calchem_hip_xc_eval_device_noalloc",
     "calchem_hip_int_create", "calchem_hip_int_release", "calchem_hip_int_one_electron", "calchem_hip_int_eri_diag",
     "calchem_hip_int_eri_columns", "calchem_hip_int_cholesky", "calchem_hip_int_jk",
+    "calchem_hip_mlp_create", "calchem_hip_mlp_release", "calchem_hip_mlp_get_params", "calchem_hip_mlp_forward",
+    "calchem_hip_mlp_loss_grad", "calchem_hip_mlp_set_data", "calchem_hip_mlp_train",
 ]
 
 _lib = None
@@ -104,6 +107,10 @@ def load() -> C.CDLL:
     L.calchem_hip_vxc_stats.argtypes = [u64, C.POINTER(VxcStats)]
     L.calchem_hip_grid_build.restype = C.c_long
     L.calchem_hip_grid_build.argtypes = [i32, vp, vp, i32, i32, i32, i32, vp, C.c_long]
+    L.calchem_hip_grid_streaming_tasks.restype = C.c_long
+    L.calchem_hip_grid_streaming_tasks.argtypes = [i32, vp, i32, i32, i32, vp, C.c_long]
+    L.calchem_hip_grid_task_points.restype = C.c_long
+    L.calchem_hip_grid_task_points.argtypes = [i32, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp, C.c_long]
     L.calchem_hip_xc_eval.argtypes = [i32, C.c_longlong, vp, vp, vp, vp]
     L.calchem_hip_xc_eval_device.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
     L.calchem_hip_xc_eval_device_noalloc.argtypes = [i32, C.c_longlong, vp, vp, vp, vp, vp]
@@ -114,6 +121,13 @@ def load() -> C.CDLL:
     L.calchem_hip_int_eri_columns.argtypes = [u64, i32, vp, vp, vp]
     L.calchem_hip_int_cholesky.argtypes = [u64, C.c_double, i32, C.POINTER(C.c_int)]
     L.calchem_hip_int_jk.argtypes = [u64, i32, vp, i32, vp, vp, vp]
+    L.calchem_hip_mlp_create.argtypes = [i32, i32, i32, vp, u64]
+    L.calchem_hip_mlp_release.argtypes = [u64]
+    L.calchem_hip_mlp_get_params.argtypes = [u64, vp]
+    L.calchem_hip_mlp_forward.argtypes = [u64, i32, vp, vp]
+    L.calchem_hip_mlp_loss_grad.argtypes = [u64, i32, vp, vp, vp, vp, vp]
+    L.calchem_hip_mlp_set_data.argtypes = [u64, i32, vp, vp, vp]
+    L.calchem_hip_mlp_train.argtypes = [u64, i32, i32, C.c_float, u64, vp]
     _lib = L
     return L
 

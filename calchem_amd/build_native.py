@@ -18,7 +18,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcalchem_hip.so")
 PROBE_LIB = os.path.join(HERE, "libcalchem_probe.so")
 PROBE_SOURCE = "probe_mfma.hip"
-SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip", "xc_points.hip", "integrals.hip"]
+SOURCES = ["api.hip", "vxc_kernels.hip", "grid.hip", "grad_kernels.hip", "xc_points.hip", "integrals.hip", "mlp.hip"]
 ERI_SOURCE = "eri_classes.hip"
 ERI_BRA_CLASSES = 6
 ARCH = os.environ.get("CALCHEM_HIP_ARCH", "gfx950")

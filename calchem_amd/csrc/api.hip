@@ -22,6 +22,9 @@
 namespace chx {
 namespace {
 
+// Radial shells per streaming task (STREAMING_RADIAL_BATCH_SHELLS, grid.rs:54).
+constexpr int kStreamingRadialShells = 8;
+
 std::atomic<size_t> g_allocated{0};
 
 struct DevBuf {
@@ -1440,6 +1443,37 @@ long calchem_hip_grid_build(int n_atoms, const int* z, const double* xyz_bohr, i
     if (!device_available()) return kNoDevice;
     int st = kOk;
     const long cnt = build_grid_host(n_atoms, z, xyz_bohr, level, n_rad, n_ang, radial_min, out, max_out, &st);
+    if (st != kOk) return st;
+    if (!out) return cnt;
+    return cnt > max_out ? kLimit : cnt;
+}
+
+long calchem_hip_grid_streaming_tasks(int n_atoms, const int* z, int level, int n_rad, int radial_min, int* tasks,
+                                      long max_tasks) {
+    if (n_atoms <= 0 || !z || level < 0 || level > 4 || (level == 4 && n_rad <= 0)) return kBadArgs;
+    long cnt = 0;
+    for (int a = 0; a < n_atoms; ++a) {
+        const int nr = atom_radial_count(z[a], level, n_rad, radial_min);
+        for (int s0 = 1; s0 <= nr; s0 += kStreamingRadialShells) {
+            if (tasks && cnt < max_tasks) {
+                tasks[3 * cnt] = a;
+                tasks[3 * cnt + 1] = s0;
+                tasks[3 * cnt + 2] = std::min(s0 + kStreamingRadialShells - 1, nr);
+            }
+            ++cnt;
+        }
+    }
+    if (tasks && cnt > max_tasks) return kLimit;
+    return cnt;
+}
+
+long calchem_hip_grid_task_points(int n_atoms, const int* z, const double* xyz_bohr, int level, int n_rad, int n_ang,
+                                  int radial_min, int atom, int rad_start, int rad_end, double* out, long max_out) {
+    if (atom < 0 || atom >= n_atoms || rad_start < 1 || rad_end < rad_start) return kBadArgs;
+    if (!device_available()) return kNoDevice;
+    int st = kOk;
+    const long cnt = build_grid_task(n_atoms, z, xyz_bohr, level, n_rad, n_ang, radial_min, atom, rad_start, rad_end,
+                                     out, max_out, &st);
     if (st != kOk) return st;
     if (!out) return cnt;
     return cnt > max_out ? kLimit : cnt;

@@ -223,92 +223,179 @@ __device__ __forceinline__ double dist_to(const double* __restrict__ xyz, int a,
     return sqrt(dx * dx + dy * dy + dz * dz);
 }
 
-// Becke cell weight of host atom h at point r with the reference's screening:
-// host product exits at 1e-20, only partners with 1 - s > 1e-17 enter the
-// normalisation, partner products exit at 1e-20.
-__global__ void k_becke(const double* __restrict__ xyz, const double* __restrict__ inv_r,
-                        const double* __restrict__ aij, int n, const double4* __restrict__ raw,
-                        const int* __restrict__ host, double* __restrict__ w_out, long long n_raw) {
-    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= n_raw) return;
-    const double4 r = raw[i];
-    const int h = host[i];
-    double bw = 1.0;
-    if (n > 1) {
-        const double rh = dist_to(xyz, h, r.x, r.y, r.z);
-        double ph = 1.0;
-        bool any = false, dead = false;
-        for (int b = 0; b < n; ++b) {
-            if (b == h) continue;
-            const double inv = inv_r[h * n + b];
-            if (inv == 0.0) continue;
-            const double mu = fmin(1.0, fmax(-1.0, (rh - dist_to(xyz, b, r.x, r.y, r.z)) * inv));
-            const double s = step3(mu + aij[h * n + b] * (1.0 - mu * mu));
-            ph *= s;
-            if (ph < 1e-20) {
-                dead = true;
-                break;
-            }
-            any |= (1.0 - s > 1e-17);
-        }
-        if (dead) {
-            bw = 0.0;
-        } else if (any) {
-            double sum = ph;
-            for (int a = 0; a < n; ++a) {
-                if (a == h) continue;
-                const double inv_ha = inv_r[h * n + a];
-                if (inv_ha == 0.0) continue;
-                const double ra = dist_to(xyz, a, r.x, r.y, r.z);
-                const double mu_h = fmin(1.0, fmax(-1.0, (rh - ra) * inv_ha));
-                const double s_ha = step3(mu_h + aij[h * n + a] * (1.0 - mu_h * mu_h));
-                if (!(1.0 - s_ha > 1e-17)) continue;
-                double pa = 1.0 - s_ha;
-                if (pa < 1e-20) continue;
-                for (int b = 0; b < n; ++b) {
-                    if (b == h || b == a) continue;
-                    const double inv = inv_r[a * n + b];
-                    if (inv == 0.0) continue;
-                    const double mu = fmin(1.0, fmax(-1.0, (ra - dist_to(xyz, b, r.x, r.y, r.z)) * inv));
-                    pa *= step3(mu + aij[a * n + b] * (1.0 - mu * mu));
-                    if (pa < 1e-20) break;
-                }
-                sum += pa;
-            }
-            bw = sum <= 0.0 ? 0.0 : ph / sum;
-        }
+// Becke cell weight of host atom h at point (x, y, z) with the reference's
+// screening: host product exits at 1e-20, only partners with 1 - s > 1e-17
+// enter the normalisation, partner products exit at 1e-20.
+__device__ double becke_cell_weight(const double* __restrict__ xyz, const double* __restrict__ inv_r,
+                                    const double* __restrict__ aij, int n, int h, double x, double y, double z) {
+    if (n <= 1) return 1.0;
+    const double rh = dist_to(xyz, h, x, y, z);
+    double ph = 1.0;
+    bool any = false;
+    for (int b = 0; b < n; ++b) {
+        if (b == h) continue;
+        const double inv = inv_r[h * n + b];
+        if (inv == 0.0) continue;
+        const double mu = fmin(1.0, fmax(-1.0, (rh - dist_to(xyz, b, x, y, z)) * inv));
+        const double s = step3(mu + aij[h * n + b] * (1.0 - mu * mu));
+        ph *= s;
+        if (ph < 1e-20) return 0.0;
+        any |= (1.0 - s > 1e-17);
     }
-    w_out[i] = r.w * bw;
+    if (!any) return 1.0;
+    double sum = ph;
+    for (int a = 0; a < n; ++a) {
+        if (a == h) continue;
+        const double inv_ha = inv_r[h * n + a];
+        if (inv_ha == 0.0) continue;
+        const double ra = dist_to(xyz, a, x, y, z);
+        const double mu_h = fmin(1.0, fmax(-1.0, (rh - ra) * inv_ha));
+        const double s_ha = step3(mu_h + aij[h * n + a] * (1.0 - mu_h * mu_h));
+        if (!(1.0 - s_ha > 1e-17)) continue;
+        double pa = 1.0 - s_ha;
+        if (pa < 1e-20) continue;
+        for (int b = 0; b < n; ++b) {
+            if (b == h || b == a) continue;
+            const double inv = inv_r[a * n + b];
+            if (inv == 0.0) continue;
+            const double mu = fmin(1.0, fmax(-1.0, (ra - dist_to(xyz, b, x, y, z)) * inv));
+            pa *= step3(mu + aij[a * n + b] * (1.0 - mu * mu));
+            if (pa < 1e-20) break;
+        }
+        sum += pa;
+    }
+    return sum <= 0.0 ? 0.0 : ph / sum;
+}
+
+// One radial shell of one atom: its centre, radius, radial weight and its
+// Lebedev rule (offset and size in the concatenated direction table); its
+// raw points start at raw_off, its kept points at keep_off.
+struct GridShell {
+    double cx, cy, cz, r, w_rad;
+    int atom, dir_off, n_dir;
+    long long raw_off, keep_off;
+};
+
+constexpr int kGridThreads = 256;
+
+// One workgroup per radial shell: points r u + centre, raw weight w_rad w_u,
+// Becke weight; raw rows (x, y, z, w, raw_w, host) and the shell's count of
+// points with w > 1e-18 (grid.rs:732-782).
+__global__ __launch_bounds__(kGridThreads) void k_grid_shell(const GridShell* __restrict__ shells,
+                                                             const double4* __restrict__ dirs,
+                                                             const double* __restrict__ xyz,
+                                                             const double* __restrict__ inv_r,
+                                                             const double* __restrict__ aij, int n_atoms,
+                                                             double* __restrict__ raw, int* __restrict__ keep) {
+    __shared__ int cnt;
+    const GridShell s = shells[blockIdx.x];
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int local = 0;
+    for (int d = threadIdx.x; d < s.n_dir; d += kGridThreads) {
+        const double4 u = dirs[s.dir_off + d];
+        const double x = s.cx + s.r * u.x, y = s.cy + s.r * u.y, z = s.cz + s.r * u.z;
+        const double raw_w = s.w_rad * u.w;
+        const double w = raw_w * becke_cell_weight(xyz, inv_r, aij, n_atoms, s.atom, x, y, z);
+        double* o = raw + 6 * (s.raw_off + d);
+        o[0] = x;
+        o[1] = y;
+        o[2] = z;
+        o[3] = w;
+        o[4] = raw_w;
+        o[5] = (double)s.atom;
+        local += w > 1e-18 ? 1 : 0;
+    }
+    atomicAdd(&cnt, local);
+    __syncthreads();
+    if (threadIdx.x == 0) keep[blockIdx.x] = cnt;
+}
+
+// One workgroup per radial shell: the kept rows, in direction order, to
+// out[keep_off ..]: per 256-direction chunk an in-order prefix of the keep
+// flags (wave ballots, then the waves in order).
+__global__ __launch_bounds__(kGridThreads) void k_grid_compact(const GridShell* __restrict__ shells,
+                                                               const double* __restrict__ raw,
+                                                               double* __restrict__ out) {
+    __shared__ int wave_cnt[kGridThreads / 64];
+    const GridShell s = shells[blockIdx.x];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    long long base = s.keep_off;
+    for (int d0 = 0; d0 < s.n_dir; d0 += kGridThreads) {
+        const int d = d0 + threadIdx.x;
+        const double* r = raw + 6 * (s.raw_off + d);
+        const bool keep = d < s.n_dir && r[3] > 1e-18;
+        const unsigned long long m = __ballot(keep);
+        const int before = __popcll(m & ((1ULL << lane) - 1ULL));
+        if (lane == 0) wave_cnt[wave] = __popcll(m);
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int w = 0; w < kGridThreads / 64; ++w) {
+            if (w < wave) off += wave_cnt[w];
+            tot += wave_cnt[w];
+        }
+        if (keep) {
+            double* o = out + 6 * (base + off + before);
+            for (int k = 0; k < 6; ++k) o[k] = r[k];
+        }
+        base += tot;
+        __syncthreads();
+    }
 }
 
 }  // namespace
 
+// Radial shell count of atom z at a level, with the basis-aware minimum and
+// Fine's 99-shell floor (grid.rs:745-752); explicit grids use n_rad.
+int atom_radial_count(int z, int level, int n_rad, int radial_min) {
+    if (level == 4) return n_rad;
+    int nr = radial_shells(z, level);
+    if (radial_min > nr) nr = radial_min;
+    if (level == 2 && nr < 99) nr = 99;
+    return nr;
+}
+
 long build_grid_host(int n_atoms, const int* z, const double* xyz, int level, int n_rad, int n_ang,
                      int radial_min, double* out, long max_out, int* status) {
+    return build_grid_task(n_atoms, z, xyz, level, n_rad, n_ang, radial_min, -1, 0, 0, out, max_out, status);
+}
+
+// Points of atom `task_atom`'s radial shells rad_start..rad_end (1-based,
+// inclusive; every atom and shell when task_atom < 0), generated, weighted
+// and filtered on the device in the reference's order (atom, radial,
+// angular).  The host only lists the radial shells.
+long build_grid_task(int n_atoms, const int* z, const double* xyz, int level, int n_rad, int n_ang, int radial_min,
+                     int task_atom, int rad_start, int rad_end, double* out, long max_out, int* status) {
     *status = kOk;
-    if (n_atoms <= 0 || !z || !xyz || level < 0 || level > 4 || (level == 4 && n_rad <= 0)) {
+    if (n_atoms <= 0 || !z || !xyz || level < 0 || level > 4 || (level == 4 && n_rad <= 0) ||
+        (task_atom >= n_atoms)) {
         *status = kBadArgs;
         return 0;
     }
-    std::vector<Dir> rules[6];
     const int sizes[6] = {26, 50, 110, 194, 302, 590};
-    for (int k = 0; k < 6; ++k) lebedev_rule(sizes[k], rules[k]);
-    std::vector<double4> raw;
-    std::vector<int> host;
+    std::vector<double4> dirs;
+    int dir_off[6];
+    for (int k = 0; k < 6; ++k) {
+        std::vector<Dir> rule;
+        lebedev_rule(sizes[k], rule);
+        dir_off[k] = (int)dirs.size();
+        for (const Dir& d : rule) dirs.push_back(make_double4(d.x, d.y, d.z, d.w));
+    }
+    std::vector<GridShell> shells;
+    long long n_raw = 0;
     for (int A = 0; A < n_atoms; ++A) {
+        if (task_atom >= 0 && A != task_atom) continue;
         const double R = bragg_slater_angstrom(z[A]) * kAngToBohr;
-        int nr = n_rad;
-        if (level != 4) {
-            nr = radial_shells(z[A], level);
-            if (radial_min > nr) nr = radial_min;
-            if (level == 2 && nr < 99) nr = 99;
-        }
+        const int nr = atom_radial_count(z[A], level, n_rad, radial_min);
+        const int i0 = task_atom >= 0 ? std::max(rad_start, 1) : 1;
+        const int i1 = task_atom >= 0 ? std::min(rad_end, nr) : nr;
         const double m = nr + 1.0;
-        for (int i = 1; i <= nr; ++i) {
+        for (int i = i0; i <= i1; ++i) {
             const double t = i, mt = m - t;
-            const double r = R * (t * t) / (mt * mt);
-            const double wr = 2.0 * ipow(R, 3) * m * ipow(t, 5) / ipow(mt, 7);
-            const int na = level == 4 ? n_ang : pruned_rule(r, R, level);
+            GridShell g;
+            g.r = R * (t * t) / (mt * mt);
+            g.w_rad = 2.0 * ipow(R, 3) * m * ipow(t, 5) / ipow(mt, 7);
+            const int na = level == 4 ? n_ang : pruned_rule(g.r, R, level);
             int k = -1;
             for (int q = 0; q < 6; ++q)
                 if (sizes[q] == na) k = q;
@@ -316,63 +403,74 @@ long build_grid_host(int n_atoms, const int* z, const double* xyz, int level, in
                 *status = kBadArgs;
                 return 0;
             }
-            for (const Dir& d : rules[k]) {
-                raw.push_back(make_double4(xyz[3 * A] + r * d.x, xyz[3 * A + 1] + r * d.y, xyz[3 * A + 2] + r * d.z,
-                                           wr * d.w));
-                host.push_back(A);
-            }
+            g.cx = xyz[3 * A];
+            g.cy = xyz[3 * A + 1];
+            g.cz = xyz[3 * A + 2];
+            g.atom = A;
+            g.dir_off = dir_off[k];
+            g.n_dir = sizes[k];
+            g.raw_off = n_raw;
+            g.keep_off = 0;
+            n_raw += sizes[k];
+            shells.push_back(g);
         }
     }
-    const long long n_raw = static_cast<long long>(raw.size());
+    if (shells.empty()) return 0;
     std::vector<double> inv_r((size_t)n_atoms * n_atoms, 0.0), aij((size_t)n_atoms * n_atoms, 0.0);
     becke_pair_tables(n_atoms, z, xyz, inv_r.data(), aij.data());
-    double *d_xyz = nullptr, *d_inv = nullptr, *d_a = nullptr, *d_w = nullptr;
-    double4* d_raw = nullptr;
-    int* d_host = nullptr;
-    std::vector<double> w(n_raw);
+    const int ns = (int)shells.size();
+    double *d_xyz = nullptr, *d_inv = nullptr, *d_a = nullptr, *d_raw = nullptr, *d_out = nullptr;
+    double4* d_dirs = nullptr;
+    GridShell* d_sh = nullptr;
+    int* d_keep = nullptr;
+    std::vector<int> keep(ns);
+    long cnt = 0;
     bool ok = hip_ok(hipMalloc(&d_xyz, sizeof(double) * 3 * n_atoms), "hipMalloc") &&
               hip_ok(hipMalloc(&d_inv, sizeof(double) * inv_r.size()), "hipMalloc") &&
               hip_ok(hipMalloc(&d_a, sizeof(double) * aij.size()), "hipMalloc") &&
-              hip_ok(hipMalloc(&d_w, sizeof(double) * (n_raw + 1)), "hipMalloc") &&
-              hip_ok(hipMalloc(&d_raw, sizeof(double4) * (n_raw + 1)), "hipMalloc") &&
-              hip_ok(hipMalloc(&d_host, sizeof(int) * (n_raw + 1)), "hipMalloc");
+              hip_ok(hipMalloc(&d_raw, sizeof(double) * 6 * n_raw), "hipMalloc") &&
+              hip_ok(hipMalloc(&d_dirs, sizeof(double4) * dirs.size()), "hipMalloc") &&
+              hip_ok(hipMalloc(&d_sh, sizeof(GridShell) * ns), "hipMalloc") &&
+              hip_ok(hipMalloc(&d_keep, sizeof(int) * ns), "hipMalloc");
     if (!ok) *status = kAllocFail;
     if (ok) {
         ok = hip_ok(hipMemcpy(d_xyz, xyz, sizeof(double) * 3 * n_atoms, hipMemcpyHostToDevice), "copy") &&
              hip_ok(hipMemcpy(d_inv, inv_r.data(), sizeof(double) * inv_r.size(), hipMemcpyHostToDevice), "copy") &&
              hip_ok(hipMemcpy(d_a, aij.data(), sizeof(double) * aij.size(), hipMemcpyHostToDevice), "copy") &&
-             hip_ok(hipMemcpy(d_raw, raw.data(), sizeof(double4) * n_raw, hipMemcpyHostToDevice), "copy") &&
-             hip_ok(hipMemcpy(d_host, host.data(), sizeof(int) * n_raw, hipMemcpyHostToDevice), "copy");
-        if (ok && n_raw > 0) {
-            hipLaunchKernelGGL(k_becke, dim3((unsigned)((n_raw + 255) / 256)), dim3(256), 0, 0, d_xyz, d_inv, d_a,
-                               n_atoms, d_raw, d_host, d_w, n_raw);
-            ok = hip_ok(hipGetLastError(), "k_becke launch") &&
-                 hip_ok(hipMemcpy(w.data(), d_w, sizeof(double) * n_raw, hipMemcpyDeviceToHost), "copy");
+             hip_ok(hipMemcpy(d_dirs, dirs.data(), sizeof(double4) * dirs.size(), hipMemcpyHostToDevice), "copy") &&
+             hip_ok(hipMemcpy(d_sh, shells.data(), sizeof(GridShell) * ns, hipMemcpyHostToDevice), "copy");
+        if (ok) {
+            hipLaunchKernelGGL(k_grid_shell, dim3(ns), dim3(kGridThreads), 0, 0, d_sh, d_dirs, d_xyz, d_inv, d_a, n_atoms,
+                               d_raw, d_keep);
+            ok = hip_ok(hipGetLastError(), "k_grid_shell launch") &&
+                 hip_ok(hipMemcpy(keep.data(), d_keep, sizeof(int) * ns, hipMemcpyDeviceToHost), "copy");
         }
-        if (!ok) *status = kLaunchFail;
-    }
-    hipFree(d_xyz);
-    hipFree(d_inv);
-    hipFree(d_a);
-    hipFree(d_w);
-    hipFree(d_raw);
-    hipFree(d_host);
-    if (!ok) return 0;
-    long cnt = 0;
-    for (long long i = 0; i < n_raw; ++i) {
-        if (!(w[i] > 1e-18)) continue;
-        if (out && cnt < max_out) {
-            double* o = out + 6 * cnt;
-            o[0] = raw[i].x;
-            o[1] = raw[i].y;
-            o[2] = raw[i].z;
-            o[3] = w[i];
-            o[4] = raw[i].w;
-            o[5] = static_cast<double>(host[i]);
+        if (ok) {
+            for (int i = 0; i < ns; ++i) {
+                shells[i].keep_off = cnt;
+                cnt += keep[i];
+            }
         }
-        ++cnt;
+        if (ok && out && cnt <= max_out && cnt > 0) {
+            ok = hip_ok(hipMalloc(&d_out, sizeof(double) * 6 * cnt), "hipMalloc") &&
+                 hip_ok(hipMemcpy(d_sh, shells.data(), sizeof(GridShell) * ns, hipMemcpyHostToDevice), "copy");
+            if (ok) {
+                hipLaunchKernelGGL(k_grid_compact, dim3(ns), dim3(kGridThreads), 0, 0, d_sh, d_raw, d_out);
+                ok = hip_ok(hipGetLastError(), "k_grid_compact launch") &&
+                     hip_ok(hipMemcpy(out, d_out, sizeof(double) * 6 * cnt, hipMemcpyDeviceToHost), "copy");
+            }
+        }
+        if (!ok && *status == kOk) *status = kLaunchFail;
     }
-    return cnt;
+    (void)hipFree(d_xyz);
+    (void)hipFree(d_inv);
+    (void)hipFree(d_a);
+    (void)hipFree(d_raw);
+    (void)hipFree(d_dirs);
+    (void)hipFree(d_sh);
+    (void)hipFree(d_keep);
+    if (d_out) (void)hipFree(d_out);
+    return ok ? cnt : 0;
 }
 
 void becke_pair_tables(int n_atoms, const int* z, const double* xyz, double* inv_r, double* aij) {

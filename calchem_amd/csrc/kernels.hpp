@@ -211,8 +211,13 @@ hipError_t launch_fill_zero(double* x, long long n, hipStream_t s);
 hipError_t launch_gather_p(const GatherArgs& a, int n_blocks, int max_mpad, hipStream_t s);
 hipError_t launch_significance(const SigArgs& a, hipStream_t s);
 
-// Grid construction (grid.hip).
+// Grid construction (grid.hip): points, Becke weights and the w > 1e-18
+// filter on the device; build_grid_task restricts to one atom's radial shells
+// rad_start..rad_end (task_atom < 0: the whole grid).
 long build_grid_host(int n_atoms, const int* z, const double* xyz, int level, int n_rad, int n_ang,
                      int radial_min, double* out, long max_out, int* status);
+long build_grid_task(int n_atoms, const int* z, const double* xyz, int level, int n_rad, int n_ang, int radial_min,
+                     int task_atom, int rad_start, int rad_end, double* out, long max_out, int* status);
+int atom_radial_count(int z, int level, int n_rad, int radial_min);
 
 }  // namespace chx

@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
+from typing import List, Optional, Tuple
 
 import numpy as np
 
@@ -42,8 +43,10 @@ def radial_min_for_basis(basis) -> int:
 
 @dataclass
 class MolecularGrid:
-    points: np.ndarray  # [n, 6] rows (x, y, z, weight, raw_weight, host_atom)
+    points: np.ndarray  # [n, 6] rows (x, y, z, weight, raw_weight, host_atom); empty when streaming
     level: str
+    # streaming grids (grid.rs:636-729): (molecule, level id, n_rad, n_ang, radial_min), points made per task
+    streaming: Optional[tuple] = None
 
     @property
     def n_points(self) -> int:
@@ -91,3 +94,47 @@ class MolecularGrid:
         if n_rad <= 0:
             raise ValueError("n_rad must be positive")
         return cls._build(mol, EXPLICIT, n_rad, n_ang, 0, f"{n_rad}x{n_ang}")
+
+    # ---- streaming grids: points generated per (atom, radial range) task on the device ----
+    @classmethod
+    def build_streaming(cls, mol: Molecule, level: str = "medium") -> "MolecularGrid":
+        """MolecularGrid::build_streaming (grid.rs:636-639): no resident points."""
+        lv = level_from_name(level)
+        return cls(np.zeros((0, 6)), lv, (mol, LEVELS[lv], 0, 0, 0))
+
+    @classmethod
+    def build_streaming_for_basis(cls, mol: Molecule, basis, level: str = "medium") -> "MolecularGrid":
+        lv = level_from_name(level)
+        return cls(np.zeros((0, 6)), lv, (mol, LEVELS[lv], 0, 0, radial_min_for_basis(basis)))
+
+    def is_streaming(self) -> bool:
+        return self.streaming is not None and self.points.shape[0] == 0
+
+    def streaming_tasks(self) -> List[Tuple[int, int, int]]:
+        """(atom, first radial shell, last radial shell) in blocks of 8 shells (grid.rs:685-704)."""
+        if self.streaming is None:
+            return []
+        mol, lv, n_rad, _, rmin = self.streaming
+        L = _native.require_device()
+        z = np.ascontiguousarray(mol.z_array(), dtype=np.int32)
+        cnt = _native.check(L.calchem_hip_grid_streaming_tasks(len(z), _native.ptr(z), lv, n_rad, rmin, None, 0),
+                            "calchem_hip_grid_streaming_tasks (count)")
+        out = np.zeros((max(cnt, 1), 3), dtype=np.int32)
+        _native.check(L.calchem_hip_grid_streaming_tasks(len(z), _native.ptr(z), lv, n_rad, rmin, _native.ptr(out),
+                                                         cnt), "calchem_hip_grid_streaming_tasks")
+        return [tuple(int(v) for v in row) for row in out[:cnt]]
+
+    def task_points(self, task: Tuple[int, int, int]) -> np.ndarray:
+        """generate_task_points (grid.rs:706-782) on the device: rows as in `points`."""
+        mol, lv, n_rad, n_ang, rmin = self.streaming
+        L = _native.require_device()
+        z = np.ascontiguousarray(mol.z_array(), dtype=np.int32)
+        xyz = np.ascontiguousarray(mol.coords_bohr().ravel(), dtype=np.float64)
+        a, s0, s1 = task
+        cnt = _native.check(L.calchem_hip_grid_task_points(len(z), _native.ptr(z), _native.ptr(xyz), lv, n_rad, n_ang,
+                                                           rmin, a, s0, s1, None, 0), "calchem_hip_grid_task_points")
+        out = np.zeros((max(cnt, 1), 6))
+        got = _native.check(L.calchem_hip_grid_task_points(len(z), _native.ptr(z), _native.ptr(xyz), lv, n_rad, n_ang,
+                                                           rmin, a, s0, s1, _native.ptr(out), cnt),
+                            "calchem_hip_grid_task_points")
+        return out[:got].copy()

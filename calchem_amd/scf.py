@@ -169,7 +169,10 @@ class RksSolver:
         return F @ P @ self.S - self.S @ P @ F
 
     def run(self, guess_density: Optional[np.ndarray] = None, guess_orbitals: Optional[np.ndarray] = None,
-            guess_name: Optional[str] = None) -> ScfResult:
+            guess_name: Optional[str] = None, superposition_guess: bool = True) -> ScfResult:
+        """One SCF.  A density guess (learned) replaces SAD; like the SAD density
+        it is not idempotent, so with superposition_guess its first Fock matrix
+        stays out of DIIS and level shifting, as SAD's does (rks.rs:296-349)."""
         o = self.opts
         n, n_occ, X = self.n, self.n_occ, self.X
         t_start = time.perf_counter()
@@ -182,6 +185,7 @@ class RksSolver:
         if P is None and guess_density is not None:
             P = np.array(guess_density, dtype=np.float64)
             gname = guess_name or "density"
+            used_sad = superposition_guess
         if P is None:
             try:
                 P = self.sad()

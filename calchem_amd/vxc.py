@@ -361,3 +361,50 @@ class GpuVxcGrid:
             self.close()
         except Exception:
             pass
+
+
+class StreamingVxcEngine:
+    """The Vxc build over a streaming grid (MolecularGrid.build_streaming,
+    grid.rs:636-729; the reference evaluates such grids task by task in
+    evaluate_vxc_rks, vxc.rs:280-384): the grid's (atom, radial range) tasks
+    are generated on the device and grouped in task order into chunks of at
+    most `max_points` points; each chunk has its own resident plan and a build
+    sums the chunks' E_xc, V and N in chunk order.  No full point list is kept
+    on the host."""
+
+    def __init__(self, basis: BasisSet, grid: MolecularGrid, max_points: int = 2_000_000, **plan_opts):
+        if not grid.is_streaming():
+            raise ValueError("StreamingVxcEngine needs a streaming grid (MolecularGrid.build_streaming)")
+        self.n_ao = basis.n_ao
+        self.engines = []
+        self.n_points = 0
+        chunk = []
+        rows = 0
+        for task in grid.streaming_tasks() + [None]:
+            pts = grid.task_points(task) if task is not None else None
+            if chunk and (pts is None or rows + pts.shape[0] > max_points):
+                sub = MolecularGrid(np.vstack(chunk), grid.level)
+                self.engines.append(RksVxcEngine(basis, sub, **plan_opts))
+                self.n_points += sub.n_points
+                chunk, rows = [], 0
+            if pts is not None and pts.shape[0]:
+                chunk.append(pts)
+                rows += pts.shape[0]
+
+    def close(self):
+        for e in self.engines:
+            e.close()
+        self.engines = []
+
+    def evaluate_full(self, func: str, P: np.ndarray, precision: str = "fp64") -> Tuple[float, np.ndarray, float]:
+        e_tot, V_tot, n_tot = 0.0, np.zeros((self.n_ao, self.n_ao)), 0.0
+        for eng in self.engines:
+            e, V, ne = eng.evaluate_full(func, P, precision)
+            e_tot += e
+            V_tot += V
+            n_tot += ne
+        return e_tot, V_tot, n_tot
+
+    def evaluate(self, basis, grid, func: str, density_total: np.ndarray, precision: str = "fp64"):
+        e, V, _ = self.evaluate_full(func, density_total, precision)
+        return e, V

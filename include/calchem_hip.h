@@ -222,13 +222,23 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats *out);
 
 /* ---- Molecular grid ----
  * Replaces MolecularGrid::build / build_for_basis (grid.rs:614-634,732-828)
- * and an explicit unpruned (n_rad x n_ang) grid.  Becke weights are computed
- * on the GPU.  out[max_out*6] rows = (x, y, z, weight, raw_weight, host_atom),
+ * and an explicit unpruned (n_rad x n_ang) grid.  Points, Becke weights and
+ * the weight filter are computed on the GPU (the host lists radial shells).  out[max_out*6] rows = (x, y, z, weight, raw_weight, host_atom),
  * in the reference's atom / radial / angular order, weight > 1e-18 only.
  * With out == NULL returns the point count; otherwise the count written, or
  * -6 when max_out is too small. */
 long calchem_hip_grid_build(int n_atoms, const int *z, const double *xyz_bohr, int level, int n_rad, int n_ang,
                             int radial_min, double *out, long max_out);
+/* Streaming grids (MolecularGrid::streaming_tasks / generate_task_points,
+ * grid.rs:685-782): the (atom, first radial shell, last radial shell) tasks of
+ * 8 radial shells each, tasks[3*k..], and the points of one task, generated
+ * and weighted on the device, rows as in calchem_hip_grid_build.  With a NULL
+ * buffer each returns the count; -6 when max_* is too small.  The tasks'
+ * points concatenated in task order are the points of calchem_hip_grid_build. */
+long calchem_hip_grid_streaming_tasks(int n_atoms, const int *z, int level, int n_rad, int radial_min, int *tasks,
+                                      long max_tasks);
+long calchem_hip_grid_task_points(int n_atoms, const int *z, const double *xyz_bohr, int level, int n_rad, int n_ang,
+                                  int radial_min, int atom, int rad_start, int rad_end, double *out, long max_out);
 
 /* ---- Integrals of the SCF Hamiltonian ----
  * The SCF row beside the Vxc build (crates/calchem-scf/src/rks.rs:131-615):
@@ -260,6 +270,25 @@ int calchem_hip_int_cholesky(uint64_t handle, double tau, int max_rank, int *ran
  * Cpos (n_ao x n_pos) and Cneg (n_ao x n_neg) row-major in host memory. */
 int calchem_hip_int_jk(uint64_t handle, int n_pos, const double *Cpos, int n_neg, const double *Cneg, double *J,
                        double *K);
+
+/* ---- The atom-pair network of the learned SCF guess ----
+ * No reference counterpart: the network replaces sad_guess at
+ * crates/calchem-scf/src/rks.rs:221-231 (SAD stays the fallback).  A
+ * perceptron d_in -> d_hidden -> d_hidden -> d_out, SiLU on the hidden layers,
+ * FP32 parameters in one array W1[h][d_in], b1[h], W2[h][h], b2[h],
+ * W3[d_out][h], b3[d_out].  Rows are samples (row-major, host memory).
+ * Loss: masked mean squared error sum mask (y - t)^2 / sum mask. */
+int calchem_hip_mlp_create(int d_in, int d_hidden, int d_out, const float *params, uint64_t handle);
+int calchem_hip_mlp_release(uint64_t handle);
+int calchem_hip_mlp_get_params(uint64_t handle, float *params);
+int calchem_hip_mlp_forward(uint64_t handle, int n, const float *x, float *y);
+/* Loss and its gradient with respect to every parameter (grad may be NULL). */
+int calchem_hip_mlp_loss_grad(uint64_t handle, int n, const float *x, const float *t, const float *mask, float *loss,
+                              float *grad);
+/* Training set resident on the device, then Adam over shuffled minibatches with
+ * a cosine learning-rate decay; loss_out[epochs] = mean minibatch loss per epoch. */
+int calchem_hip_mlp_set_data(uint64_t handle, int n, const float *x, const float *t, const float *mask);
+int calchem_hip_mlp_train(uint64_t handle, int epochs, int batch, float lr, uint64_t seed, float *loss_out);
 
 /* ---- The functional on caller points ----
  * No reference entry point: evaluates eval_xc_spin (crates/calchem-dft/src/xc.rs:159-223)
