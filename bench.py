@@ -125,6 +125,11 @@ def measured_mfma_peak(prec: str, n_wg: int = 2048, iters: int = 2000) -> float:
     return out.value
 
 
+def plan_extra(args) -> dict:
+    """Plan options beyond the launch shapes: the mixed threshold and the AO cache."""
+    return {"mixed_exact_threshold": args.mixed_threshold, "ao_cache": args.ao_cache}
+
+
 def sources_digest() -> str:
     """sha256 over the kernel and plan sources: a traffic summary applies only to the code it was measured on."""
     h = hashlib.sha256()
@@ -224,11 +229,12 @@ def bench_batched(args) -> int:
         f"(total {sum(n_ao)}), {n_pts} points (setup {time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     beng = BatchedVxcEngine([(b, g) for b, g, _ in systems], args.block_points, args.group_points,
-                            rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows)
+                            rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows,
+                       **plan_extra(args))
     t_plan_b = time.perf_counter() - t0
     t0 = time.perf_counter()
     engines = [RksVxcEngine(b, g, args.block_points, args.group_points, rho_tile_points=args.rho_tile_points,
-                            group_max_rows=args.group_max_rows) for b, g, _ in systems]
+                            group_max_rows=args.group_max_rows, **plan_extra(args)) for b, g, _ in systems]
     t_plan_s = time.perf_counter() - t0
     st = beng.stats()
     log(f"plans: batched {t_plan_b:.1f}s ({st['n_blocks']} blocks, {st['n_groups']} groups, flops rho "
@@ -314,6 +320,10 @@ def main() -> int:
     ap.add_argument("--group-points", type=int, default=0)
     ap.add_argument("--rho-tile-points", type=int, default=0, choices=[0, 16])
     ap.add_argument("--group-max-rows", type=int, default=0, choices=[0, 192, 272])
+    ap.add_argument("--mixed-threshold", type=float, default=0.0,
+                    help="max |phi| above which the mixed build keeps an AO row FP64 (0 = library default)")
+    ap.add_argument("--ao-cache", type=int, default=0, choices=[0, 1, 2],
+                    help="mixed build's AO cache: 0 auto, 1 off, 2 on")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
     args = ap.parse_args()
@@ -344,12 +354,14 @@ def main() -> int:
         f"(setup {time.perf_counter() - t0:.1f}s)")
     t0 = time.perf_counter()
     eng = RksVxcEngine(basis, grid, args.block_points, args.group_points,
-                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows)
+                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows,
+                       **plan_extra(args))
     t_plan = time.perf_counter() - t0
     st0 = eng.stats()
     log(f"plan {t_plan:.2f}s: {st0['n_blocks']} blocks, {st0['n_groups']} groups, rho tiles "
         f"{st0['n_rho_tiles']}, V pairs {st0['n_v_pairs']}, max m {st0['max_m_rho']}/{st0['max_m_v']}, "
-        f"flops rho {st0['flops_rho']:.3e} v {st0['flops_v']:.3e}, device {st0['device_bytes'] / 2**30:.2f} GiB")
+        f"flops rho {st0['flops_rho']:.3e} v {st0['flops_v']:.3e}, device {st0['device_bytes'] / 2**30:.2f} GiB "
+        f"(AO cache {st0['ao_cache_bytes'] / 2**30:.2f} GiB)")
 
     n = basis.n_ao
     stream = hr.Stream()
@@ -405,7 +417,8 @@ def main() -> int:
 
     # Unrestricted build of the same workload (100 alpha / 99 beta orbitals), device buffers.
     uks = UksVxcEngine(basis, grid, args.block_points, args.group_points,
-                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows)
+                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows,
+                       **plan_extra(args))
     Pa = 0.5 * P
     Pb = 0.5 * configs.random_density(basis, w["n_occ"] - 1, w["seed"] + 1, S)
     dPa, dPb = hr.DeviceArray.from_numpy(Pa), hr.DeviceArray.from_numpy(Pb)
@@ -514,7 +527,8 @@ def main() -> int:
         "config": {"workload": f"{args.workload}: {mol.n_atoms} atoms, 6-31G(d,p) {n} AO, "
                                f"{w['n_rad']}x{w['n_ang']} Becke grid ({grid.n_points} points), "
                                f"{func.upper()}, random S-orthonormal P ({w['n_occ']} occ)",
-                   "precision": best, "func": func, "n_ao": n, "n_points": grid.n_points},
+                   "precision": best, "func": func, "n_ao": n, "n_points": grid.n_points,
+                   "ao_cache_bytes": st0["ao_cache_bytes"]},
         "fp64_builds_per_s": 1000.0 / r64["ms"],
         "fp32_builds_per_s": 1000.0 / r32["ms"],
         "mixed_vs_fp64_speedup": r64["ms"] / r32["ms"],

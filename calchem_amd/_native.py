@@ -29,7 +29,8 @@ class CalchemHipError(RuntimeError):
 class VxcOptions(C.Structure):
     _fields_ = [("block_points", C.c_int32), ("group_points", C.c_int32),
                 ("memory_limit_bytes", C.c_size_t), ("mixed_exact_threshold", C.c_double),
-                ("rho_tile_points", C.c_int32), ("group_max_rows", C.c_int32), ("reserved", C.c_int32 * 2)]
+                ("rho_tile_points", C.c_int32), ("group_max_rows", C.c_int32), ("ao_cache", C.c_int32),
+                ("reserved", C.c_int32)]
 
 
 class VxcStats(C.Structure):
@@ -43,7 +44,7 @@ class VxcStats(C.Structure):
                 ("lds_bytes_per_block", C.c_size_t),
                 ("profiled_builds", C.c_int64), ("ms_gather", C.c_double), ("ms_rho", C.c_double),
                 ("ms_xc", C.c_double), ("ms_vxc", C.c_double), ("ms_reduce_v", C.c_double), ("ms_reduce_en", C.c_double),
-                ("flops_rho_fp64_rows", C.c_double)]
+                ("flops_rho_fp64_rows", C.c_double), ("ao_cache_bytes", C.c_size_t)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -78,7 +78,7 @@ struct Plan {
     std::vector<char> class_fits[2];
     bool v_fits_fp32 = true;  // the mixed build's V kernel fits the LDS limit
     double flops_rho = 0, flops_v = 0, flops_rho_pad = 0, flops_v_pad = 0, flops_rho_fp64_rows = 0;
-    double mixed_threshold = 0.25;
+    double mixed_threshold = 0.1;
     size_t lds_limit = 0;
     int n_group_list = 0, reduce_waves = 1;
     int reduce_segments = 1;  // workgroups per AO row in the V reduction (fewer rows than CUs x 8: more)
@@ -96,7 +96,8 @@ struct Plan {
         group_sh, group_list, pairs, ao_gbegin, ao_groups, scratch, wv, wv_caller, epart, npart, en, P64, Pb_host,
         V, Vb_host, rho, rho_s, psub64, psub32, psub64b, psub32b, class_blocks, rho_sb, wv_b, mols, ao_mol, xc_wg,
         shell_atom,
-        atom_xyz, inv_r, asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum;
+        atom_xyz, inv_r, asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum,
+        aoc, aoc_off;  // the mixed build's AO cache (empty: AOs evaluated per build)
     int n_atoms = 0;  // atoms set by calchem_hip_vxc_set_atoms (0 = not set)
     bool uks_ready = false;  // rho_sb / wv_b / psub64b / psub32b allocated (first unrestricted build)
     long long psub_elems = 0;
@@ -127,7 +128,7 @@ struct Plan {
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &psub64b, &psub32b, &class_blocks, &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
-                             &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum};
+                             &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum, &aoc, &aoc_off};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
         return s;
@@ -312,6 +313,8 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (want_pts == 32) return kUnsupported;  // the rho stage walks blocks 16 points at a time
     const int want_rows = opt ? opt->group_max_rows : 0;
     if (want_rows != 0 && want_rows != 192 && want_rows != 272) return kBadArgs;
+    const int aoc_mode = opt ? opt->ao_cache : 0;
+    if (aoc_mode < 0 || aoc_mode > 2) return kBadArgs;
     P.group_max_tiles = want_rows == 192 ? 12 : kGroupMaxTiles;
 
     // Device shell / AO tables.
@@ -721,6 +724,24 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                         sizeof(double4) * (size_t)(2 * P.n_pts + n_pts) + 12 * (size_t)P.psub_elems;
     if (mem_limit > 0 && need > mem_limit) return kLimit;
 
+    // The mixed build's AO cache: per rho workgroup tile, its sub-tiles' slots
+    // in the kernel's lane order (aoc_sub_bytes).  Offsets in 16-byte units.
+    std::vector<long long> haoc_off(htiles.size());
+    size_t aoc_bytes = 0;
+    for (size_t t = 0; t < htiles.size(); ++t) {
+        const RhoBlock& B = hb[htiles[t].x];
+        const int n_sub = std::min(kRhoSub * kRhoPts, B.n_pts - htiles[t].y) > kRhoPts ? 2 : 1;
+        haoc_off[t] = static_cast<long long>(aoc_bytes >> 4);
+        aoc_bytes += static_cast<size_t>(n_sub * aoc_sub_bytes(B.m_pad, B.m64));
+    }
+    bool want_aoc = aoc_mode != 1 && aoc_bytes > 0;
+    if (want_aoc && aoc_mode == 0) {
+        size_t free_b = 0, total_b = 0;
+        want_aoc = hipMemGetInfo(&free_b, &total_b) == hipSuccess && aoc_bytes <= free_b / 4 &&
+                   (mem_limit == 0 || need + aoc_bytes <= mem_limit);
+    }
+    if (want_aoc && aoc_mode == 2 && mem_limit > 0 && need + aoc_bytes > mem_limit) return kLimit;
+
 
     std::vector<int> hperm(P.n_pts);
     for (int i = 0; i < P.n_pts; ++i) hperm[i] = keep[i];
@@ -751,6 +772,24 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             hipEventCreateWithFlags(&P.join_ev[i], hipEventDisableTiming) != hipSuccess)
             return kNoDevice;
     if (hipEventCreateWithFlags(&P.fork_ev, hipEventDisableTiming) != hipSuccess) return kNoDevice;
+    if (want_aoc) {
+        if (!upload(P.aoc_off, haoc_off) || !P.aoc.alloc(aoc_bytes)) {
+            (void)hipGetLastError();  // clear the failed allocation's error
+            P.aoc_off.reset();
+            P.aoc.reset();
+            if (aoc_mode == 2) return kAllocFail;
+        } else {
+            // one evaluation per geometry, by the mixed rho kernel's own AO code
+            RhoArgs fa{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
+                       P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(),
+                       P.block_sh.as<int2>(), P.block_meta.as<uint4>(), nullptr, nullptr, nullptr, nullptr, 0};
+            fa.aoc = P.aoc.p;
+            fa.aoc_off = P.aoc_off.as<long long>();
+            for (size_t c = 0; c < P.classes.size(); ++c)
+                if (P.class_fits[kRhoMixed][c] && launch_rho_fill_cache(fa, P.classes[c], P.stream) != hipSuccess)
+                    return kLaunchFail;
+        }
+    }
     if (hipDeviceSynchronize() != hipSuccess) return kLaunchFail;
     std::lock_guard<std::mutex> lk(g_table_mu);
     g_table[id] = plan;
@@ -841,6 +880,10 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
                    P.rho_tiles.as<int2>(), P.blocks.as<RhoBlock>(), P.block_rows.as<int2>(), P.block_sh.as<int2>(),
                    P.block_meta.as<uint4>(), dP64, dPS, sorted ? sorted : P.rho.as<double4>(), P.perm.as<int>(),
                    sorted ? 0 : 1};
+        if (mode == kRhoMixed && P.aoc.p) {
+            ra.aoc = P.aoc.p;
+            ra.aoc_off = P.aoc_off.as<long long>();
+        }
         if (!two) return launch_rho(ra, mode, cl, st) == hipSuccess ? kOk : kLaunchFail;
         // both spins from one AO evaluation when the class's LDS allows,
         // else one pass per spin
@@ -1427,6 +1470,7 @@ int calchem_hip_vxc_stats(uint64_t handle, CalchemVxcStats* out) {
     out->rho_points_fp32 = kRhoPts;
     out->flops_rho_fp64_rows = P.flops_rho_fp64_rows;
     out->device_bytes = P.device_bytes();
+    out->ao_cache_bytes = P.aoc.bytes;
     out->lds_bytes_per_block = P.lds_limit;
     out->profiled_builds = P.prof.builds;
     out->ms_gather = P.prof.ms[0];

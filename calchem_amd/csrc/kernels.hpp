@@ -24,7 +24,24 @@ struct RhoArgs {
     const double* P64b;
     const void* PSb;
     double4* rho_out_b;
+    // Per-geometry AO cache of the mixed build (null: AOs evaluated per
+    // build).  aoc_off[t] is tile t's offset in 16-byte units; see
+    // aoc_tile_bytes for the layout.
+    void* aoc;
+    const long long* aoc_off;
 };
+
+// AO cache of the mixed rho stage: per rho workgroup tile, per 16-point
+// sub-tile, per 16-row tile of the block, the (phi, dphi/dx, dphi/dy,
+// dphi/dz) of the 64 lanes x 4 slots exactly as the rho kernel's lanes hold
+// them (slot v of lane (j, q): row Mma<float>::row(q, v) of the row tile,
+// point j): [v][lane] of double4 for the block's leading row tiles (rows kept
+// FP64), of float4 for the others.
+constexpr long long kAocRowTile64 = 64LL * 4 * 32;  // bytes per leading row tile
+constexpr long long kAocRowTile32 = 64LL * 4 * 16;  // bytes per FP32 row tile
+__host__ __device__ inline long long aoc_sub_bytes(int m_pad, int m64) {
+    return (long long)(m64 >> 4) * kAocRowTile64 + (long long)((m_pad - m64) >> 4) * kAocRowTile32;
+}
 
 // Functional-stage workgroups never straddle molecules: workgroup w covers
 // sorted points [wg[w].x, wg[w].x + wg[w].y).
@@ -190,8 +207,12 @@ struct SigArgs {
 // Rho kernel variants: FP64 throughout, or mixed (leading rows FP64, the
 // rest FP32).
 enum RhoMode { kRhoFp64 = 0, kRhoMixed = 1 };
-size_t rho_lds_bytes(int mode, const RhoClass& c, int ns = 1);
+size_t rho_lds_bytes(int mode, const RhoClass& c, int ns = 1, bool cached = false);
+// Mixed launches read the AO cache when a.aoc is set.
 hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s, int ns = 1);
+// Fill a.aoc for one launch class (mixed layout): the AO evaluation of the
+// mixed rho kernel, stored instead of contracted.
+hipError_t launch_rho_fill_cache(const RhoArgs& a, const RhoClass& c, hipStream_t s);
 hipError_t launch_xc(const XcArgs& a, hipStream_t s);
 hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);

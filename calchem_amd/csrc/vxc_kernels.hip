@@ -77,10 +77,7 @@ struct Vec4<float> {
 // multiple of 16 bytes): points, per-wave sums, the exponential's table,
 // radial table [shell][point],
 // rows, shells, primitives (the block's packed records, copied as one run),
-// FP64 leading-row values, row values.
-__host__ __device__ constexpr size_t rho_lds_fixed(int ns) {
-    return sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 16 * ns * kRhoPts + sizeof(double) * 64;
-}
+// FP64 leading-row values, row values (rho_lds_bytes).
 
 // Workgroups are dispatched round robin over the XCDs (workgroup i to XCD
 // i mod kXcds), each with its own L2.  Workgroup i = kXcds k + x takes tile k
@@ -107,8 +104,16 @@ __device__ __forceinline__ int f64_row_as_f32(int i) { return 4 * (i & 3) + (i >
 // NS = 2 (unrestricted builds): the same AO values serve both spin densities;
 // each row tile is contracted with the alpha and then the beta density block
 // and both sets of sums are formed (rho_out gets alpha, rho_out_b beta).
-template <typename TS, bool HYB, int NS>
+//
+// AOC: kAocNone evaluates the AOs; kAocFill evaluates them and stores them
+// in the per-geometry cache a.aoc (no contraction, no output); kAocLoad reads
+// them from the cache instead of evaluating (mixed build only: the leading
+// rows' values are FP64 there, the others FP32).
+enum AocMode { kAocNone = 0, kAocFill = 1, kAocLoad = 2 };
+template <typename TS, bool HYB, int NS, int AOC>
 __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
+    static_assert(AOC == kAocNone || (HYB && sizeof(TS) == 4), "the AO cache has the mixed layout");
+    static_assert(AOC != kAocFill || NS == 1, "one fill pass per geometry");
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = 2;
     const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
@@ -117,21 +122,24 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     const RhoBlock blk = a.blocks[tile.x];
     const int m = blk.m, mp = blk.m_pad, nsh = blk.n_sh, T16 = blk.m_pad >> 4;
     const int m64 = HYB ? blk.m64 : 0;
+    // (a launch that reads the AO cache stages no points, records or radial
+    // sums: those pieces are empty)
+    constexpr bool kEval = AOC != kAocLoad;
     unsigned char* q = smem;
     double4* sPtAll = reinterpret_cast<double4*>(q);  // [kRhoSub * PTS]
-    q += sizeof(double4) * kRhoSub * kRhoPts;
+    q += kEval ? sizeof(double4) * kRhoSub * kRhoPts : 0;
     double* sRed = reinterpret_cast<double*>(q);  // [4 waves][4 NS sums][PTS]
     q += sizeof(double) * 16 * NS * PTS;
     double* sExp = reinterpret_cast<double*>(q);  // [64] the exponential's table
-    q += sizeof(double) * 64;
+    q += kEval ? sizeof(double) * 64 : 0;
     double2* sRad = reinterpret_cast<double2*>(q);  // [nsh][PTS] (R0, R1)
-    q += sizeof(double2) * nsh * PTS;
+    q += kEval ? sizeof(double2) * nsh * PTS : 0;
     RowL* sRow = reinterpret_cast<RowL*>(q);
-    q += sizeof(RowL) * mp;
+    q += kEval ? sizeof(RowL) * mp : 0;
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
-    q += sizeof(DevShell) * nsh;
+    q += kEval ? sizeof(DevShell) * nsh : 0;
     double2* sPrim = reinterpret_cast<double2*>(q);
-    q += sizeof(double2) * blk.n_prim;
+    q += kEval ? sizeof(double2) * blk.n_prim : 0;
     double* sPhi64 = reinterpret_cast<double*>(q);  // [m64][LDP]
     q += HYB ? sizeof(double) * m64 * LDP : 0;
     TS* sPhi = reinterpret_cast<TS*>(q);  // [mp][LDP]
@@ -149,7 +157,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     V4 pre0{}, pre1{};
     double pre64[4] = {0.0, 0.0, 0.0, 0.0};
     auto preload = [&]() {
-        if (wave < T16) {
+        if (AOC != kAocFill && wave < T16) {
             const TS* __restrict__ Arow = PS + blk.psub + static_cast<long long>((wave << 4) + col) * mp;
             pre0 = *reinterpret_cast<const V4*>(Arow + m64 + 4 * kq);
             pre1 = *reinterpret_cast<const V4*>(Arow + m64 + 16 + 4 * kq);
@@ -162,7 +170,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         }
     };
     preload();
-    {
+    if constexpr (AOC != kAocLoad) {
         // rows, shells and primitives: one coalesced copy of the packed records
         const uint4* __restrict__ src = a.meta + blk.meta;
         uint4* dst = reinterpret_cast<uint4*>(sRow);
@@ -173,9 +181,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     // The workgroup's points: kRhoSub sub-tiles of 16, taken one after the
     // other with the block's records staged once.
     const int t0 = tile.y, np_all = min(kRhoSub * PTS, blk.n_pts - t0);
-    if (tid < kRhoSub * PTS)
+    if (AOC != kAocLoad && tid < kRhoSub * PTS)
         sPtAll[tid] = tid < np_all ? a.pts[blk.pt_begin + t0 + tid] : make_double4(0.0, 0.0, 0.0, 0.0);
     __syncthreads();
+    // This workgroup's part of the AO cache (see aoc_sub_bytes).
+    const int L16 = m64 >> 4;
+    unsigned char* const cache_base =
+        AOC != kAocNone ? static_cast<unsigned char*>(a.aoc) + (a.aoc_off[t] << 4) : nullptr;
+    const long long sub_bytes = aoc_sub_bytes(mp, m64);
     // (each sub-tile is a separate inlined copy of the body, not a loop:
     // a loop lets the compiler hoist per-tile addresses across both passes
     // and nearly doubles the registers)
@@ -184,8 +197,44 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         const int np = min(PTS, np_all - sub * PTS), tsub = t0 + sub * PTS;
         if (sub > 0) preload();
 
+        // Slot v of row tile it of this sub-tile in the AO cache.
+        unsigned char* const csub = AOC != kAocNone ? cache_base + sub * sub_bytes : nullptr;
+        auto cslot = [&](int it, int v) {
+            return it < L16 ? csub + it * kAocRowTile64 + (v * 64 + lane) * 32
+                            : csub + L16 * kAocRowTile64 + (it - L16) * kAocRowTile32 + (v * 64 + lane) * 16;
+        };
+        // The four slots of row tile it: raw words first, so that several
+        // row tiles' loads are in flight before any is used.  Both halves are
+        // loaded unconditionally (a branch here makes every load wait): a
+        // leading row tile's slot fills both, an FP32 slot the first and
+        // reads itself again into the second, which stays unused.
+        auto craw = [&](int it, uint4 (&w)[4][2]) {
+            const int second = it < L16 ? 1 : 0;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const uint4* p = reinterpret_cast<const uint4*>(cslot(it, v));
+                w[v][0] = p[0];
+                w[v][1] = p[second];
+            }
+        };
+        auto cunpack = [&](int it, const uint4 (&w)[2]) {
+            if (it < L16)
+                return AoVal{__hiloint2double(w[0].y, w[0].x), __hiloint2double(w[0].w, w[0].z),
+                             __hiloint2double(w[1].y, w[1].x), __hiloint2double(w[1].w, w[1].z)};
+            return AoVal{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z),
+                         __uint_as_float(w[0].w)};
+        };
+        auto cstore = [&](int it, int v, const AoVal& f) {
+            if (it < L16)
+                *reinterpret_cast<double4*>(cslot(it, v)) = make_double4(f.v, f.gx, f.gy, f.gz);
+            else
+                *reinterpret_cast<float4*>(cslot(it, v)) =
+                    make_float4(static_cast<float>(f.v), static_cast<float>(f.gx), static_cast<float>(f.gy),
+                                static_cast<float>(f.gz));
+        };
+
         // Radial sums once per (shell, point).
-        for (int idx = tid; idx < nsh * PTS; idx += kThreads) {
+        for (int idx = tid; AOC != kAocLoad && idx < nsh * PTS; idx += kThreads) {
             const int sh = idx / PTS, p = idx % PTS;
             double2 rr = make_double2(0.0, 0.0);
             if (p < np) {
@@ -212,7 +261,25 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             }
             return f;
         };
+        if constexpr (AOC == kAocFill) {
+            for (int it = wave; it < T16; it += 4)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) cstore(it, v, eval(it, v));
+            return;
+        }
         AoVal cache[kRhoCached][4];
+        if constexpr (AOC == kAocLoad) {
+            // (a wave without a j-th row tile loads the block's last one
+            // again: unconditional loads, none of them waited on early)
+            uint4 w[kRhoCached][4][2];
+#pragma unroll
+            for (int j = 0; j < kRhoCached; ++j) craw(min(wave + 4 * j, T16 - 1), w[j]);
+#pragma unroll
+            for (int j = 0; j < kRhoCached; ++j)
+                if (wave + 4 * j < T16)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) cache[j][v] = cunpack(wave + 4 * j, w[j][v]);
+        }
 #pragma unroll
         for (int j = 0; j < kRhoCached; ++j) {
             const int it = wave + 4 * j;
@@ -220,17 +287,19 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int r = (it << 4) + M::row(kq, v);
-                    cache[j][v] = eval(it, v);
+                    if constexpr (AOC != kAocLoad) cache[j][v] = eval(it, v);
                     sPhi[r * LDP + col] = static_cast<TS>(cache[j][v].v);
                     if (HYB && r < m64) sPhi64[r * LDP + col] = cache[j][v].v;
                 }
             }
         }
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
+            uint4 w[4][2];
+            if constexpr (AOC == kAocLoad) craw(it, w);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = (it << 4) + M::row(kq, v);
-                const double fv = eval(it, v).v;
+                const double fv = AOC == kAocLoad ? cunpack(it, w[v]).v : eval(it, v).v;
                 sPhi[r * LDP + col] = static_cast<TS>(fv);
                 if (HYB && r < m64) sPhi64[r * LDP + col] = fv;
             }
@@ -338,11 +407,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
             typename M::acc acc[NS];
             v4d acc64[NS];
+            uint4 w[4][2];
+            if constexpr (AOC == kAocLoad) craw(it, w);
 #pragma unroll
             for (int sp = 0; sp < NS; ++sp) contract(it, acc[sp], acc64[sp], false, sp);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const AoVal f = eval(it, v);
+                const AoVal f = AOC == kAocLoad ? cunpack(it, w[v]) : eval(it, v);
 #pragma unroll
                 for (int sp = 0; sp < NS; ++sp) accumulate(sp, acc[sp], acc64[sp], v, f);
             }
@@ -1180,16 +1251,16 @@ __global__ void k_significance(SigArgs a) {
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
-template <typename TS, bool HYB, int NS>
+template <typename TS, bool HYB, int NS, int AOC = kAocNone>
 static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
-    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, c, NS);
+    const size_t lds = rho_lds_bytes(HYB ? kRhoMixed : kRhoFp64, c, NS, AOC == kAocLoad);
     static int attr_set = 0;
     if (!attr_set) {
         int dev = 0, lim = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, HYB, NS>),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho<TS, HYB, NS, AOC>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, lim);
         if (e != hipSuccess) return e;
         attr_set = 1;
@@ -1197,28 +1268,40 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
     if (c.n_tiles <= 0) return hipSuccess;
     RhoArgs b = a;
     b.tiles = a.tiles + c.tile_begin;
+    if (AOC != kAocNone) b.aoc_off = a.aoc_off + c.tile_begin;
     int longest = 0;
     for (int x = 0; x < kXcds; ++x) {
         b.seg[x] = c.seg[x];
         longest = std::max(longest, c.seg[x + 1] - c.seg[x]);
     }
     b.seg[kXcds] = c.seg[kXcds];
-    hipLaunchKernelGGL((k_rho<TS, HYB, NS>), dim3(kXcds * longest), dim3(kThreads), lds, s, b);
+    hipLaunchKernelGGL((k_rho<TS, HYB, NS, AOC>), dim3(kXcds * longest), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
-size_t rho_lds_bytes(int mode, const RhoClass& c, int ns) {
+size_t rho_lds_bytes(int mode, const RhoClass& c, int ns, bool cached) {
     const size_t ts = mode == kRhoMixed ? sizeof(float) : sizeof(double);
-    return rho_lds_fixed(ns) + sizeof(double2) * (size_t)c.max_nsh * kRhoPts + sizeof(DevShell) * (size_t)c.max_nsh +
-           sizeof(double2) * (size_t)c.max_prim + sizeof(RowL) * (size_t)c.max_mpad +
+    const size_t staged = cached ? 0
+                                 : sizeof(double4) * kRhoSub * kRhoPts + sizeof(double) * 64 +
+                                       sizeof(double2) * (size_t)c.max_nsh * kRhoPts +
+                                       sizeof(DevShell) * (size_t)c.max_nsh + sizeof(double2) * (size_t)c.max_prim +
+                                       sizeof(RowL) * (size_t)c.max_mpad;
+    return sizeof(double) * 16 * (size_t)ns * kRhoPts + staged +
            (mode == kRhoMixed ? sizeof(double) * (size_t)c.max_m64 * kRhoPts : 0) +
            ts * (size_t)c.max_mpad * kRhoPts;
 }
 
 hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t s, int ns) {
-    if (ns == 2)
-        return mode == kRhoFp64 ? launch_rho_t<double, false, 2>(a, c, s) : launch_rho_t<float, true, 2>(a, c, s);
-    return mode == kRhoFp64 ? launch_rho_t<double, false, 1>(a, c, s) : launch_rho_t<float, true, 1>(a, c, s);
+    if (mode == kRhoFp64)
+        return ns == 2 ? launch_rho_t<double, false, 2>(a, c, s) : launch_rho_t<double, false, 1>(a, c, s);
+    if (a.aoc)
+        return ns == 2 ? launch_rho_t<float, true, 2, kAocLoad>(a, c, s) : launch_rho_t<float, true, 1, kAocLoad>(a, c, s);
+    return ns == 2 ? launch_rho_t<float, true, 2>(a, c, s) : launch_rho_t<float, true, 1>(a, c, s);
+}
+
+hipError_t launch_rho_fill_cache(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
+    if (!a.aoc || !a.aoc_off) return hipErrorInvalidValue;
+    return launch_rho_t<float, true, 1, kAocFill>(a, c, s);
 }
 
 // The kernel instance of functional a.func (an XcId); unknown ids are

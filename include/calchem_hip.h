@@ -77,10 +77,16 @@ typedef struct {
     int32_t block_points;          /* max points per rho-stage block (default 256) */
     int32_t group_points;          /* target points per V-stage group (default 384) */
     size_t memory_limit_bytes;     /* 0 = no limit */
-    double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.25) */
+    double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.1) */
     int32_t rho_tile_points;       /* points per rho tile: 0 or 16 (other values: -1 / -7) */
     int32_t group_max_rows;        /* AO rows per register-resident V group: 0 = default (272), 192 or 272 */
-    int32_t reserved[2];
+    /* Per-geometry AO cache of the mixed build's rho stage (AO values and
+     * gradients at every point, the leading rows FP64, the others FP32, about
+     * 16 B per (significant AO, point)): 0 = when it fits a quarter of the
+     * free device memory (and memory_limit_bytes), 1 = never, 2 = always (-3
+     * / -6 when it cannot be had).  FP64 builds never use it. */
+    int32_t ao_cache;
+    int32_t reserved;
 } CalchemVxcOptions;
 
 typedef struct {
@@ -101,6 +107,7 @@ typedef struct {
     double ms_reduce_v;
     double ms_reduce_en;
     double flops_rho_fp64_rows;      /* part of flops_rho over the rows a mixed build keeps in FP64 */
+    size_t ao_cache_bytes;           /* the mixed build's AO cache (0: none) */
 } CalchemVxcStats;
 
 /* ---- Device queries: crates/calchem-cuda/src/lib.rs:469-489 (is_cuda_available,

@@ -18,8 +18,11 @@ the 1e-7 Eh gate).  Variants:
                 split computes
   bf16x3        three BF16 terms each, six products (all with i + j <= 2)
   hybrid_bf16x3 hybrid rows FP64, the rest bf16x3
+  cache_tau     hybrid_tau, and the epilogue (rho = sum phi T, grad rho) also
+                takes phi and grad phi of the non-leading rows rounded to FP32:
+                what a per-geometry AO cache holding those rows in FP32 gives
 
-    python oracle/precision_study.py [--stride 4]
+    python oracle/precision_study.py [--stride 4] [--cluster N SEED --grid NR NA --occ N SEED --func F]
 
 (--stride k keeps every k-th 128-point batch of the grid; dE_xc is scaled by
 k, an estimate of the whole grid's.)
@@ -93,15 +96,24 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--stride", type=int, default=4)
     ap.add_argument("--tau", type=float, nargs="*", default=[0.25, 2.0])
+    ap.add_argument("--cluster", type=int, nargs=2, default=None, metavar=("N_WATERS", "SEED"),
+                    help="a seeded water cluster instead of the 20-water workload")
+    ap.add_argument("--grid", type=int, nargs=2, default=None, metavar=("N_RAD", "N_ANG"))
+    ap.add_argument("--occ", type=int, nargs=2, default=None, metavar=("N_OCC", "SEED"),
+                    help="occupied orbitals and seed of the random density")
+    ap.add_argument("--func", default="b3lyp")
+    ap.add_argument("--variants", nargs="*", default=None)
     a = ap.parse_args()
     w = configs.WORKLOADS["water20"]
-    mol = w["build"]()
+    mol = configs.water_cluster(*a.cluster) if a.cluster else w["build"]()
     basis = BasisSet.from_name(w["basis"], mol)
-    P = configs.random_density(basis, w["n_occ"], w["seed"])
-    pts = O.build_grid(mol, "explicit", w["n_rad"], w["n_ang"])
+    n_occ, seed = a.occ if a.occ else (w["n_occ"], w["seed"])
+    P = configs.random_density(basis, n_occ, seed)
+    n_rad, n_ang = a.grid if a.grid else (w["n_rad"], w["n_ang"])
+    pts = O.build_grid(mol, "explicit", n_rad, n_ang)
     ob = O.OracleBasis(basis)
-    variants = ["fp32", "fp32_phi", "fp32_p", "split2_fp32", "bf16x2", "bf16x3", "hybrid_bf16x3"] + \
-               [f"hybrid_{t:g}" for t in a.tau]
+    variants = a.variants or (["fp32", "fp32_phi", "fp32_p", "split2_fp32", "bf16x2", "bf16x3", "hybrid_bf16x3"] +
+                              [f"hybrid_{t:g}" for t in a.tau] + [f"cache_{t:g}" for t in a.tau])
     dE = {v: 0.0 for v in variants}
     t0 = time.time()
     nb = (len(pts) + 127) // 128
@@ -124,23 +136,29 @@ def main():
         for g in range(len(sl)):
             if rho[g] >= 1e-18:
                 gam = 0.25 * float(grho[:, g] @ grho[:, g])
-                o = O.eval_xc("b3lyp", 0.5 * rho[g], 0.5 * rho[g], gam, gam, gam)
+                o = O.eval_xc(a.func, 0.5 * rho[g], 0.5 * rho[g], gam, gam, gam)
                 vr[g] = 0.5 * (o[1] + o[2])                 # d f / d rho
                 vg[g] = 0.25 * (o[3] + o[4] + o[5])         # d f / d |grad rho|^2
         for v in variants:
             lead = None
-            if v.startswith("hybrid"):
+            if v.startswith("hybrid") or v.startswith("cache"):
                 tau = 0.25 if v.endswith("bf16x3") else float(v.split("_")[1])
                 lead = np.abs(phi).max(axis=1) > tau
-            Tv = contract(Ps, phi, v, lead)
-            # the epilogue forms rho and grad rho with the FP64 AO values
-            rho_v = np.sum(phi * Tv, axis=0)
-            grho_v = 2.0 * np.sum(dphi * Tv[None], axis=1)
+            Tv = contract(Ps, phi, "hybrid_" + v.split("_")[1] if v.startswith("cache") else v, lead)
+            ph, dph = phi, dphi
+            if v.startswith("cache"):
+                # the non-leading rows' AO values and gradients as FP32
+                ph = np.where(lead[:, None], phi, phi.astype(np.float32).astype(np.float64))
+                dph = np.where(lead[None, :, None], dphi, dphi.astype(np.float32).astype(np.float64))
+            # the epilogue forms rho and grad rho with these AO values
+            rho_v = np.sum(ph * Tv, axis=0)
+            grho_v = 2.0 * np.sum(dph * Tv[None], axis=1)
             d = ww * (vr * (rho_v - rho) + 2.0 * vg * np.sum(grho * (grho_v - grho), axis=0))
             dE[v] += float(np.sum(d))
         if b % (50 * a.stride) == 0:
             print(f"batch {b}/{nb} ({time.time() - t0:.0f}s)", flush=True)
-    print(f"20 waters, B3LYP, {nb} batches, every {a.stride}: estimated dE_xc of the whole grid")
+    print(f"{mol.n_atoms // 3} waters, {a.func}, {n_rad}x{n_ang}, {nb} batches, every {a.stride}: "
+          f"estimated dE_xc of the whole grid")
     for v in variants:
         print(f"  {v:15s} {a.stride * dE[v]: .2e} Eh")
 

@@ -3,8 +3,9 @@
 (the k_vxc_group / k_vxc_group_pipe path the throughput figure times).
 
 FP64 build against the oracle on every point: |dE_xc| < 1e-10 Eh,
-max|dV| < 1e-9, |dN| < 1e-9 and exact symmetry; the mixed build against the
-FP64 build: |dE_xc| < 1e-7 Eh (the mixed-precision gate).
+max|dV| < 1e-9, |dN| < 1e-9 and exact symmetry; the mixed build (with its
+per-geometry AO cache, the default) against the FP64 build: |dE_xc| < 1e-7 Eh
+(the mixed-precision gate).
 """
 import numpy as np
 import pytest
@@ -29,6 +30,7 @@ def test_water20_75x302_default_shapes():
     eng = RksVxcEngine(basis, grid)
     st = eng.stats()
     assert st["n_v_pairs"] == 0  # every group register-resident: the default V kernels
+    assert st["ao_cache_bytes"] > 0  # the mixed build reads the per-geometry AO cache
     e, V, ne = eng.evaluate_full("b3lyp", P, "fp64")
     e32, V32, n32 = eng.evaluate_full("b3lyp", P, "fp32")
     e_ref, V_ref, ne_ref = O.vxc_rks(O.OracleBasis(basis), grid.points, "b3lyp", P)
