@@ -1187,10 +1187,12 @@ __global__ void k_to_float(const double* __restrict__ x, float* __restrict__ y, 
 
 // One workgroup per block: its m_pad x m_pad density sub-block, four rows
 // per wave at a time (wave w takes rows 4 w .. 4 w + 3, then 16 further, and
-// so on; lanes run along the rows, so the stores are contiguous, and the
-// four rows' loads of a column chunk are in flight together).  The block's
-// AO indices are staged in LDS first (dynamic LDS of 4 x the largest m_pad
-// of the launch).
+// so on; lanes run along the rows, so the stores are contiguous).  Each
+// thread requests its four rows x four column chunks (up to 256 columns) of
+// loads together, before any store: the loads read P through clamped
+// indices (padding rows and columns read P[0][0]) and padding is zeroed
+// after, so no load sits behind a branch.  The block's AO indices are staged
+// in LDS first (dynamic LDS of 4 x the largest m_pad of the launch).
 __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
     extern __shared__ int sIdx[];
     const RhoBlock B = a.blocks[a.block_list ? a.block_list[blockIdx.x] : static_cast<int>(blockIdx.x)];
@@ -1202,23 +1204,32 @@ __global__ __launch_bounds__(kThreads) void k_gather_p(GatherArgs a) {
         sIdx[r] = ao >= 0 ? ao - mol.ao_base : -1;
     }
     __syncthreads();
-    constexpr int R = 4;
+    constexpr int R = 4, C = 4;
     for (int i0 = wave * R; i0 < mp; i0 += R * (kThreads / 64)) {  // mp is a multiple of 16
         int ri[R];
 #pragma unroll
         for (int u = 0; u < R; ++u) ri[u] = sIdx[i0 + u];
-        for (int j = lane; j < mp; j += 64) {
-            const int rj = sIdx[j];
-            double v[R];
+        for (int j0 = lane; j0 < mp; j0 += 64 * C) {
+            int rj[C];
+#pragma unroll
+            for (int c = 0; c < C; ++c) rj[c] = j0 + 64 * c < mp ? sIdx[j0 + 64 * c] : -1;
+            double v[R][C];
 #pragma unroll
             for (int u = 0; u < R; ++u)
-                v[u] = (ri[u] >= 0 && rj >= 0) ? P[static_cast<long long>(ri[u]) * mol.n_ao + rj] : 0.0;
 #pragma unroll
-            for (int u = 0; u < R; ++u) {
-                const long long o = B.psub + static_cast<long long>(i0 + u) * mp + j;
-                if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[o] = v[u];
-                if (a.out32) a.out32[o] = static_cast<float>(v[u]);
-            }
+                for (int c = 0; c < C; ++c)
+                    v[u][c] = P[static_cast<long long>(max(ri[u], 0)) * mol.n_ao + max(rj[c], 0)];
+#pragma unroll
+            for (int u = 0; u < R; ++u)
+#pragma unroll
+                for (int c = 0; c < C; ++c) {
+                    const int j = j0 + 64 * c;
+                    if (j >= mp) continue;
+                    const double x = (ri[u] >= 0 && rj[c] >= 0) ? v[u][c] : 0.0;
+                    const long long o = B.psub + static_cast<long long>(i0 + u) * mp + j;
+                    if (a.out64 && (!a.out64_leading_only || j < B.m64)) a.out64[o] = x;
+                    if (a.out32) a.out32[o] = static_cast<float>(x);
+                }
         }
     }
 }
