@@ -75,6 +75,7 @@ struct Plan {
     int max_mpad_rho = 0, max_m_v = 0;
     // Rho launch classes and, per mode (FP64 / mixed), whether each fits LDS.
     std::vector<RhoClass> classes;
+    std::vector<int> class_order;  // classes by decreasing cost: the order of the rho stage's launches
     std::vector<char> class_fits[2];
     bool v_fits_fp32 = true;  // the mixed build's V kernel fits the LDS limit
     double flops_rho = 0, flops_v = 0, flops_rho_pad = 0, flops_v_pad = 0, flops_rho_fp64_rows = 0;
@@ -547,6 +548,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
                 const int x = static_cast<int>(std::min_element(load, load + kXcds) - load);
                 seg[x].push_back(b);
                 load[x] += block_cost(b);
+                cls[c].cost += block_cost(b);
             }
             cls[c].block_begin = static_cast<int>(hclsblk.size());
             cls[c].n_blocks = static_cast<int>(bl.size());
@@ -566,6 +568,12 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
             cls[c].n_tiles = cls[c].seg[kXcds];
             P.classes.push_back(cls[c]);
         }
+        // the rho stage starts the costliest class first, on the build's own
+        // stream, and the others beside it in decreasing cost
+        P.class_order.resize(P.classes.size());
+        for (size_t c = 0; c < P.classes.size(); ++c) P.class_order[c] = static_cast<int>(c);
+        std::stable_sort(P.class_order.begin(), P.class_order.end(),
+                         [&](int x, int y) { return P.classes[x].cost > P.classes[y].cost; });
     }
 
     // V groups: consecutive blocks (k-d order) merged up to nb_v points and,
@@ -872,8 +880,8 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
     const double* dP64 = P.psub64.as<double>();
     const void* dPS = mode == kRhoMixed ? static_cast<const void*>(P.psub32.p) : P.psub64.p;
     // the LDS classes cover disjoint tiles: side by side (their tails overlap)
-    return fan_out(P, s, static_cast<int>(P.classes.size()), [&](int c, hipStream_t st) {
-        const RhoClass& cl = P.classes[c];
+    return fan_out(P, s, static_cast<int>(P.classes.size()), [&](int i, hipStream_t st) {
+        const RhoClass& cl = P.classes[P.class_order[i]];
         if (per_class && !gather(P.class_blocks.as<int>() + cl.block_begin, cl.n_blocks, cl.max_mpad, st))
             return kLaunchFail;
         RhoArgs ra{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),

@@ -210,7 +210,22 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
 // a, a + 64, ...; per-atom sums stay in registers over the run and go to
 // this wave's partial row at the end.  The host atom gets minus the sum over
 // the other atoms' terms of each point.
+//
+// Per point, after the cell functions: an atom c is active when its largest
+// leave-one-out product (the product of all its factors s_cd but the
+// smallest) exceeds kBeckeScreen Z, Z the sum of the cell functions.  A pair
+// (a, b) enters the derivative through (P_a / s_ab - P_b / s_ba) ds_ab/dR_a,
+// and each leave-one-out product of an inactive atom is at most
+// kBeckeScreen Z: what is dropped per pair and point is at most
+// 2 kBeckeScreen raw |f_xc| |ds_ab/dR_a| <= 2e-20 x 6.75 / R_ab raw |f_xc|
+// (|ds/dnu| <= 27/16, |1 - 2 a mu| <= 2).  Pairs of two inactive atoms are
+// skipped; pairs with an active second atom b run in full with lanes over
+// the first atom a; pairs of an active first atom and an inactive second
+// one keep the P_a / s_ab term, with lanes over b, summed across the lanes
+// in a fixed order.  About 10 of 60 atoms are active at a point of the
+// 20-water grid.
 constexpr int kBeckeAtomsPerLane = 4;  // up to 256 atoms (APL = 1, 2 or 4 atoms per lane)
+constexpr double kBeckeScreen = 1e-20;
 
 __device__ __forceinline__ void becke_f3_deriv(double nu, double& f3, double& ds) {
     const double x = fmin(fmax(nu, -1.0), 1.0);
@@ -234,6 +249,8 @@ __device__ __forceinline__ double recip(double x) {
     return fma(r, fma(-x, r, 1.0), r);
 }
 
+size_t grad_becke_lds_bytes(int n_atoms) { return (sizeof(double) * 5 + sizeof(int) * 4) * (size_t)n_atoms; }
+
 template <int APL>
 __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -242,8 +259,11 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
     double* sCell = sDist + n;                        // [n] P_c = prod_{d != c} s_cd
     double* sBig = sCell + n;                         // [n] product of the factors s_cd > 1e-12
     double* sTiny = sBig + n;                         // [n] product of the nonzero factors s_cd <= 1e-12
-    int* sNTiny = reinterpret_cast<int*>(sTiny + n);  // [n] how many factors are <= 1e-12
+    double* sMax = sTiny + n;                         // [n] largest leave-one-out product
+    int* sNTiny = reinterpret_cast<int*>(sMax + n);   // [n] how many factors are <= 1e-12
     int* sNZero = sNTiny + n;                         // [n] how many factors are exactly 0
+    int* sAct = sNZero + n;                           // [n] the active atoms, ascending
+    int* sIsAct = sAct + n;                           // [n] 1 for an active atom
     double acc[APL][3];
 #pragma unroll
     for (int k = 0; k < APL; ++k) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
@@ -259,11 +279,12 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
             sDist[c] = sqrt(dx * dx + dy * dy + dz * dz);
         }
         __syncthreads();
-        // cell function P_c = prod_{d != c} s_cd, and the same product split
-        // into its factors above and at or below 1e-12 (for the products that
-        // leave one factor out)
+        // cell function P_c = prod_{d != c} s_cd, the same product split into
+        // its factors above and at or below 1e-12 (for the products that
+        // leave one factor out), and the product of all factors but the
+        // smallest (the largest of those products)
         for (int c = lane; c < n; c += 64) {
-            double pc = 1.0, big = 1.0, tiny = 1.0;
+            double pc = 1.0, big = 1.0, tiny = 1.0, smin = 1.0, pmax = 1.0;
             int nt = 0, nz = 0;
             for (int d = 0; d < n; ++d) {
                 if (d == c) continue;
@@ -276,6 +297,12 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
                 double s, ds;
                 becke_step_deriv(nu, s, ds);
                 pc *= s;
+                if (s < smin) {
+                    pmax *= smin;
+                    smin = s;
+                } else {
+                    pmax *= s;
+                }
                 if (s > 1e-12) {
                     big *= s;
                 } else {
@@ -289,6 +316,7 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
             sCell[c] = pc;
             sBig[c] = big;
             sTiny[c] = tiny;
+            sMax[c] = pmax;
             sNTiny[c] = nt;
             sNZero[c] = nz;
         }
@@ -296,67 +324,142 @@ __global__ __launch_bounds__(64) void k_grad_becke(GradBeckeArgs a) {
         double zs = 0.0;
         for (int c = 0; c < n; ++c) zs += sCell[c];
         if (zs > 1e-30) {
+            // the active atoms, ascending (ballot prefix per 64 atoms)
+            const double thr = kBeckeScreen * zs;
+            int n_act = 0;
+#pragma unroll
+            for (int k = 0; k < APL; ++k) {
+                const int c = lane + 64 * k;
+                const bool act = c < n && sMax[c] > thr;
+                const unsigned long long m = __ballot(act);
+                if (act) sAct[n_act + __popcll(m & ((1ull << lane) - 1ull))] = c;
+                if (c < n) sIsAct[c] = act ? 1 : 0;
+                n_act += __popcll(m);
+            }
+            __syncthreads();
             const double wn = sCell[host] / zs, pref = raw * fxc / zs;
-            double hsum[3] = {0.0, 0.0, 0.0};
+            // products over the other pairs of c: P / s when s is above 1e-12;
+            // else from the split products, in which s is one of the factors at
+            // or below 1e-12 (and one of the zeros if 0)
+            auto except = [&](int c, double sv) {
+                if (sv > 1e-12) return sCell[c] * recip(sv);
+                if (sNTiny[c] == 1) return sBig[c];
+                if (sv > 0.0) return sNZero[c] > 0 ? 0.0 : sBig[c] * (sTiny[c] / sv);
+                return sNZero[c] > 1 ? 0.0 : sBig[c] * sTiny[c];
+            };
+            // the terms of pair (at, b) for atom at (ra = |r - R_at| >= 1e-14,
+            // dra = d ra / dR_at) into dz (dZ / dR_at) and dp (dP_host / dR_at)
+            auto pair = [&](int at, double ra, const double (&dra)[3], int b, double (&dz)[3], double (&dp)[3]) {
+                const double inv = a.inv_r[b * n + at];
+                if (inv == 0.0) return;
+                const double mu = fmin(fmax((ra - sDist[b]) * inv, -1.0), 1.0);
+                const double asy = -a.asym[b * n + at];
+                const double nu = mu + asy * (1.0 - mu * mu);
+                double f3, dsdnu;
+                becke_f3_deriv(nu, f3, dsdnu);
+                const double s_ab = 0.5 * (1.0 - f3);
+                const double slope = dsdnu * (1.0 - 2.0 * asy * mu);
+                if (fabs(slope) < 1e-30) return;
+                const double rv[3] = {a.xyz[3 * at] - a.xyz[3 * b], a.xyz[3 * at + 1] - a.xyz[3 * b + 1],
+                                      a.xyz[3 * at + 2] - a.xyz[3 * b + 2]};
+                // s_ba as the cell loop formed it for atom b (so that it is one
+                // of the factors counted there): its nu is exactly -nu, f3 is
+                // odd, so s_ba = (1 + f3) / 2 bit for bit
+                const double s_ba = 0.5 * (1.0 + f3);
+                double p_ab, p_ba;
+                if (s_ab > 1e-12 && s_ba > 1e-12) {
+                    // both leave-one-out products by division, from one reciprocal
+                    const double r = recip(s_ab * s_ba);
+                    p_ab = sCell[at] * (s_ba * r);
+                    p_ba = sCell[b] * (s_ab * r);
+                } else {
+                    p_ab = except(at, s_ab);
+                    p_ba = except(b, s_ba);
+                }
+                // dsab_c = slope (dra_c - mu rv_c / R) / R, with t = slope / R and
+                // u = mu / R formed once per pair
+                const double t = slope * inv, u = mu * inv, dpp = p_ab - p_ba;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const double dsab = t * fma(-u, rv[c], dra[c]);
+                    dz[c] += dpp * dsab;
+                    if (b == host) dp[c] -= p_ba * dsab;
+                }
+            };
+            // the same for a second atom b that is inactive: its leave-one-out
+            // product p_ba is at most kBeckeScreen Z, so only p_ab ds_ab/dR_at
+            // is kept (and nothing for P_host when b is the host)
+            auto pair_ab = [&](int at, double ra, const double (&dra)[3], int b, double (&dz)[3]) {
+                const double inv = a.inv_r[b * n + at];
+                if (inv == 0.0) return;
+                const double mu = fmin(fmax((ra - sDist[b]) * inv, -1.0), 1.0);
+                const double asy = -a.asym[b * n + at];
+                const double nu = mu + asy * (1.0 - mu * mu);
+                double f3, dsdnu;
+                becke_f3_deriv(nu, f3, dsdnu);
+                const double slope = dsdnu * (1.0 - 2.0 * asy * mu);
+                if (fabs(slope) < 1e-30) return;
+                const double p_ab = except(at, 0.5 * (1.0 - f3));
+                const double t = slope * inv * p_ab, u = mu * inv;
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    dz[c] += t * fma(-u, a.xyz[3 * at + c] - a.xyz[3 * b + c], dra[c]);
+            };
+            auto geometry = [&](int at, double& ra, double (&dra)[3]) {
+                ra = sDist[at];
+                dra[0] = -(p.x - a.xyz[3 * at]) / ra;
+                dra[1] = -(p.y - a.xyz[3 * at + 1]) / ra;
+                dra[2] = -(p.z - a.xyz[3 * at + 2]) / ra;
+            };
+            double dzs[APL][3], dph[APL][3], ra[APL], dra[APL][3];
+            bool ok[APL];
 #pragma unroll
             for (int k = 0; k < APL; ++k) {
                 const int at = lane + 64 * k;
-                if (at >= n || at == host) continue;
-                const double ra = sDist[at];
-                if (ra < 1e-14) continue;
-                const double dra[3] = {-(p.x - a.xyz[3 * at]) / ra, -(p.y - a.xyz[3 * at + 1]) / ra,
-                                       -(p.z - a.xyz[3 * at + 2]) / ra};
-                double dph[3] = {0.0, 0.0, 0.0}, dzs[3] = {0.0, 0.0, 0.0};
-                for (int b = 0; b < n; ++b) {
-                    if (b == at) continue;
-                    const double inv = a.inv_r[b * n + at];
-                    if (inv == 0.0) continue;
-                    const double mu = fmin(fmax((ra - sDist[b]) * inv, -1.0), 1.0);
-                    const double asy = -a.asym[b * n + at];
-                    const double nu = mu + asy * (1.0 - mu * mu);
-                    double f3, dsdnu;
-                    becke_f3_deriv(nu, f3, dsdnu);
-                    const double s_ab = 0.5 * (1.0 - f3);
-                    const double slope = dsdnu * (1.0 - 2.0 * asy * mu);
-                    if (fabs(slope) < 1e-30) continue;
-                    const double rv[3] = {a.xyz[3 * at] - a.xyz[3 * b], a.xyz[3 * at + 1] - a.xyz[3 * b + 1],
-                                          a.xyz[3 * at + 2] - a.xyz[3 * b + 2]};
-                    // s_ba as the cell loop formed it for atom b (so that it is one
-                    // of the factors counted there): its nu is exactly -nu, f3 is
-                    // odd, so s_ba = (1 + f3) / 2 bit for bit
-                    const double s_ba = 0.5 * (1.0 + f3);
-                    // products over the other pairs of at and of b: P / s when s is
-                    // above 1e-12; else from the split products, in which s is one
-                    // of the factors at or below 1e-12 (and one of the zeros if 0)
-                    auto except = [&](int c, double sv) {
-                        if (sv > 1e-12) return sCell[c] * recip(sv);
-                        if (sNTiny[c] == 1) return sBig[c];
-                        if (sv > 0.0) return sNZero[c] > 0 ? 0.0 : sBig[c] * (sTiny[c] / sv);
-                        return sNZero[c] > 1 ? 0.0 : sBig[c] * sTiny[c];
-                    };
-                    double p_ab, p_ba;
-                    if (s_ab > 1e-12 && s_ba > 1e-12) {
-                        // both leave-one-out products by division, from one reciprocal
-                        const double r = recip(s_ab * s_ba);
-                        p_ab = sCell[at] * (s_ba * r);
-                        p_ba = sCell[b] * (s_ab * r);
-                    } else {
-                        p_ab = except(at, s_ab);
-                        p_ba = except(b, s_ba);
-                    }
-                    // dsab_c = slope (dra_c - mu rv_c / R) / R, with t = slope / R and
-                    // u = mu / R formed once per pair
-                    const double t = slope * inv, u = mu * inv, dp = p_ab - p_ba;
+                ok[k] = at < n && at != host && sDist[at] >= 1e-14;
+                dzs[k][0] = dzs[k][1] = dzs[k][2] = 0.0;
+                dph[k][0] = dph[k][1] = dph[k][2] = 0.0;
+                ra[k] = 1.0;
+                dra[k][0] = dra[k][1] = dra[k][2] = 0.0;
+                if (ok[k]) geometry(at, ra[k], dra[k]);
+            }
+            // pairs whose second atom is active: lanes over the first
+            for (int j = 0; j < n_act; ++j) {
+                const int b = sAct[j];
 #pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const double dsab = t * fma(-u, rv[c], dra[c]);
-                        dzs[c] += dp * dsab;
-                        if (b == host) dph[c] -= p_ba * dsab;
-                    }
+                for (int k = 0; k < APL; ++k)
+                    if (ok[k] && b != lane + 64 * k) pair(lane + 64 * k, ra[k], dra[k], b, dzs[k], dph[k]);
+            }
+            // pairs of an active first atom and an inactive second one: lanes
+            // over the second, summed across the lanes (xor order), then onto
+            // the first atom's lane
+            for (int j = 0; j < n_act; ++j) {
+                const int at = __builtin_amdgcn_readfirstlane(sAct[j]);
+                if (at == host || sDist[at] < 1e-14) continue;  // uniform
+                double rat, drat[3];
+                geometry(at, rat, drat);
+                double pz[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k = 0; k < APL; ++k) {
+                    const int b = lane + 64 * k;
+                    if (b < n && b != at && !sIsAct[b]) pair_ab(at, rat, drat, b, pz);
                 }
 #pragma unroll
+                for (int c = 0; c < 3; ++c)
+                    for (int o = 32; o > 0; o >>= 1) pz[c] += __shfl_xor(pz[c], o);
+#pragma unroll
+                for (int k = 0; k < APL; ++k)
+                    if (lane + 64 * k == at)
+#pragma unroll
+                        for (int c = 0; c < 3; ++c) dzs[k][c] += pz[c];
+            }
+            double hsum[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int k = 0; k < APL; ++k) {
+                if (!ok[k]) continue;
+#pragma unroll
                 for (int c = 0; c < 3; ++c) {
-                    const double dwf = pref * (dph[c] - wn * dzs[c]);
+                    const double dwf = pref * (dph[k][c] - wn * dzs[k][c]);
                     acc[k][c] += dwf;
                     hsum[c] += dwf;
                 }
@@ -427,7 +530,7 @@ hipError_t launch_grad_ao(const GradAoArgs& a, int n_tiles, size_t lds, hipStrea
 
 hipError_t launch_grad_becke(const GradBeckeArgs& a, int n_waves, hipStream_t s) {
     if (n_waves <= 0) return hipSuccess;
-    const size_t lds = (sizeof(double) * 4 + sizeof(int) * 2) * (size_t)a.n_atoms;
+    const size_t lds = grad_becke_lds_bytes(a.n_atoms);
     if (a.n_atoms <= 64)
         hipLaunchKernelGGL(k_grad_becke<1>, dim3(n_waves), dim3(64), lds, s, a);
     else if (a.n_atoms <= 128)

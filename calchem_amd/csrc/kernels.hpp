@@ -134,6 +134,7 @@ struct RhoClass {
     int max_mpad, max_nsh, max_m64, max_prim;
     int seg[kXcds + 1];
     int block_begin, n_blocks;  // the class's blocks: range in the plan's class-ordered block list
+    double cost;                // sum of its tiles' costs (m_pad^2 plus a fixed part per tile)
 };
 
 struct VArgs {
