@@ -91,6 +91,23 @@ def test_uks_mixed_gate_c8h18():
     assert abs(e32 - e64) < 1e-7 and dv < 1e-4 and abs(n32 - n64) < 1e-7
 
 
+def test_uks_mixed_half_densities_equal_rks_mixed():
+    """Mixed UKS(P/2, P/2): Va and Vb bitwise equal (the two spins run the same
+    kernels on the same inputs) and within the mixed build's own V error of
+    the mixed RKS(P) build (max|dV| < 1e-6)."""
+    w = configs.WORKLOADS["c8h18"]
+    mol = w["build"]()
+    basis = BasisSet.from_name(w["basis"], mol)
+    grid = MolecularGrid.explicit(mol, w["n_rad"], w["n_ang"])
+    P = configs.random_density(basis, w["n_occ"], w["seed"])
+    eng = UksVxcEngine(basis, grid)
+    e_u, Va, Vb, n_u = eng.evaluate_full("b3lyp", 0.5 * P, 0.5 * P, "fp32")
+    e_r, V_r, n_r = RksVxcEngine.evaluate_full(eng, "b3lyp", P, "fp32")
+    print(f"uks/rks mixed: dE={e_u - e_r:.2e} dVa={np.abs(Va - V_r).max():.2e} dN={n_u - n_r:.2e}")
+    assert np.array_equal(Va, Vb)
+    assert abs(e_u - e_r) < 1e-9 and np.abs(Va - V_r).max() < 1e-6 and abs(n_u - n_r) < 1e-9
+
+
 def test_uks_device_entry_matches_host_entry():
     """calchem_hip_vxc_build_uks_device on device buffers gives the host-buffer result."""
     from calchem_amd import hip_runtime as hr
