@@ -28,6 +28,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Vxc builds/s + % MFMA peak, mixed vs FP64 GPU at |dE_xc|<1e-7 Eh"
 PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+# Full-rate VALU issue of the whole chip: 256 CUs x 4 SIMDs, one wave64
+# instruction per SIMD every 4 cycles (16 lanes a cycle) at 2.4 GHz.
+PEAK_VALU_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 E_GATE = 1e-7
 
 
@@ -151,6 +154,47 @@ def load_traffic(workload: str, precision: str, kernel: str):
             or t.get("sources_sha256") != sources_digest()):
         return None
     return t
+
+
+def load_valu(workload: str):
+    """VALU instructions per build of the stages' kernels (SQ_INSTS_VALU minus SQ_INSTS_MFMA) from the
+    committed rocprofv3 PMC summary profiles/valu_<workload>.json, if it was measured on these sources."""
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    if t.get("workload") != workload or t.get("sources_sha256") != sources_digest():
+        return None
+    return t
+
+
+def valu_rooflines(workload: str, kms_by_prec: dict):
+    """VALU roofline of the AO-evaluation and functional stages: counted instructions per build over the
+    stage time measured in this run, against the full-rate VALU issue of the chip."""
+    t = load_valu(workload)
+    if t is None:
+        return None
+    out = {}
+    for name, e in t["stages"].items():
+        ms = kms_by_prec[e["precision"]][e["stage"]]
+        achieved = e["valu_instr_per_build"] / (ms * 1e-3)
+        out[name] = {"kernels": e["kernels"], "instr_per_build": e["valu_instr_per_build"], "stage_ms": ms,
+                     "achieved": achieved, "peak": PEAK_VALU_INSTR_PER_S, "unit": "wave64 VALU instr/s",
+                     "frac": achieved / PEAK_VALU_INSTR_PER_S}
+    out["source"] = t["source"]
+    return out
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(basis, grid, func, P, target_s, threads):
@@ -497,7 +541,7 @@ def main() -> int:
             sample = (f"{cb['batches']} of {cb['n_batches']} 128-point batches in 8 evenly spaced runs "
                       f"({cb['seconds']:.1f}s), scaled to a full build")
         cpu = {"value": cb["builds_per_s"], "unit": "builds/s", "cores": threads, "kind": "port",
-               "sample": sample}
+               "sample": sample, "cpu_model": cpu_model()}
         # FP64 GPU parity with the oracle on exactly the sampled points.
         idx = np.concatenate([np.arange(a * 128, min(b * 128, grid.n_points)) for a, b in cb["ranges"]])
         sub = MolecularGrid(grid.points[idx].copy(), grid.level)
@@ -541,6 +585,7 @@ def main() -> int:
         "scf": scf,
         "build_tflops": build_tflops,
         "kernel_ms": kms,
+        "valu_roofline": valu_rooflines(args.workload, {"fp64": r64["kernel_ms"], "fp32": r32["kernel_ms"]}),
         "roofline": {"bound": "mfma", "kernel": kname, "launches": "all launches of one build",
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "measured_peak": measured_peak, "frac_of_measured_peak": achieved / measured_peak,
