@@ -115,7 +115,11 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     static_assert(AOC == kAocNone || (HYB && sizeof(TS) == 4), "the AO cache has the mixed layout");
     static_assert(AOC != kAocFill || NS == 1, "one fill pass per geometry");
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = 2;
+    // row tiles per wave whose (phi, grad phi) stay in registers for the
+    // epilogue: two when they are evaluated (later ones are evaluated again),
+    // one when they come from the AO cache (later ones are read again, from
+    // L2), which keeps that kernel at 120 VGPRs: four waves per SIMD
+    constexpr int PTS = kRhoPts, LDP = kRhoPts, kRhoCached = AOC == kAocLoad ? 1 : 2;
     const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
     if (t >= a.seg[xcd + 1]) return;
     const int2 tile = a.tiles[t];  // (block, first point within it): up to kRhoSub x 16 points
