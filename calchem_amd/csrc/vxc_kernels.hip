@@ -464,6 +464,215 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     }
 }
 
+// Mixed rho stage from the AO cache, one spin, with the workgroup's two
+// 16-point sub-tiles contracted together: every density chunk of
+// T = P_sub Phi that a wave loads feeds the MFMAs of both sub-tiles (32
+// points), so the density blocks are read half as often as by one sub-tile
+// at a time.  LDS holds the B operands of both sub-tiles: Phi of the FP64
+// leading rows (below m64) and of the FP32 rows (from m64 on).  Wave w takes
+// the 16-row tiles w, w + 4, ...; the epilogue reads each tile's (phi, grad
+// phi) of both sub-tiles from the cache a second time (from L2): for an FP32
+// row tile before its contraction, so that the loads arrive during it, for
+// a leading row tile (32-byte slots) after it.  The sums are those of
+// k_rho<float, true, 1, kAocLoad>, formed in the same order.
+__global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int PTS = kRhoPts, PT2 = 2 * kRhoPts;
+    const int xcd = blockIdx.x % kXcds, t = a.seg[xcd] + blockIdx.x / kXcds;
+    if (t >= a.seg[xcd + 1]) return;
+    const int2 tile = a.tiles[t];  // (block, first point within it): up to 32 points
+    const RhoBlock blk = a.blocks[tile.x];
+    const int mp = blk.m_pad, T16 = mp >> 4, m64 = blk.m64, L16 = m64 >> 4, mr = mp - m64;
+    const int np_all = min(PT2, blk.n_pts - tile.y);
+    const bool two = np_all > PTS;  // the second sub-tile exists (and has its cache slots)
+    double* sRed = reinterpret_cast<double*>(smem);                   // [4 waves][4 sums][PT2]
+    double* sPhi64 = sRed + 16 * PT2;                                  // [2 sub-tiles][m64][PTS]
+    float* sPhi = reinterpret_cast<float*>(sPhi64 + 2 * m64 * PTS);    // [2 sub-tiles][mr][PTS]
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int col = lane & 15, kq = lane >> 4;
+    // the block's density blocks: uniform bases, 32-bit per-lane offsets
+    const float* __restrict__ PSb = static_cast<const float*>(a.PS) + blk.psub;
+    const double* __restrict__ P64b = a.P64 + blk.psub;
+    // Slot v of row tile it of sub-tile s in the AO cache (see aoc_sub_bytes).
+    const unsigned char* const cbase = static_cast<const unsigned char*>(a.aoc) + (a.aoc_off[t] << 4);
+    const long long sub_bytes = aoc_sub_bytes(mp, m64);
+    auto slot = [&](int it, int v, int s) {
+        const unsigned char* c = cbase + s * sub_bytes;
+        return it < L16 ? c + it * kAocRowTile64 + static_cast<unsigned>((v * 64 + lane) * 32)
+                        : c + L16 * kAocRowTile64 + (it - L16) * kAocRowTile32 + static_cast<unsigned>((v * 64 + lane) * 16);
+    };
+    // B operands: phi of both sub-tiles (a missing second sub-tile is zero).
+    for (int it = wave; it < T16; it += 4) {
+        if (it < L16) {
+            double f[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    f[s][v] = s == 0 || two ? *reinterpret_cast<const double*>(slot(it, v, s)) : 0.0;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) sPhi64[(s * m64 + (it << 4) + 4 * kq + v) * PTS + col] = f[s][v];
+        } else {
+            float f[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    f[s][v] = s == 0 || two ? *reinterpret_cast<const float*>(slot(it, v, s)) : 0.f;
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    sPhi[(s * mr + (it << 4) - m64 + 4 * kq + v) * PTS + col] = f[s][v];
+        }
+    }
+    __syncthreads();
+
+    // sums[s] = (rho, grad rho / 2) at this lane's point of sub-tile s
+    double sums[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sums[s][k] = 0.0;
+    auto accumulate = [&](int s, const v4f& acc, const v4d& acc64, int v, const AoVal& f) {
+        const double tv = static_cast<double>(acc[v]) + acc64[v];
+        sums[s][0] += f.v * tv;
+        sums[s][1] += f.gx * tv;
+        sums[s][2] += f.gy * tv;
+        sums[s][3] += f.gz * tv;
+    };
+    // T of row tile it for both sub-tiles: this lane ends up with
+    // T[4 kq + v][col] of sub-tile s in acc[s][v] + acc64[s][v].
+    auto contract = [&](int it, v4f (&acc)[2], v4d (&acc64)[2]) {
+        const int i0 = it << 4;
+        acc[0] = acc[1] = Mma<float>::zero();
+        acc64[0] = acc64[1] = Mma<double>::zero();
+        const unsigned o32 = static_cast<unsigned>((i0 + col) * mp + m64 + 4 * kq);
+        auto load = [&](int c) { return *reinterpret_cast<const v4f*>(PSb + (o32 + 16u * c)); };
+        v4f r0 = load(0);
+        v4f r1 = load(1);
+        __builtin_amdgcn_sched_barrier(0);
+        // leading columns in FP64 with the FP64 density block, four k-steps
+        // per round with their loads issued together
+        const unsigned o64 = static_cast<unsigned>((i0 + f64_row_as_f32(col)) * mp + kq);
+        for (int k = 0; k < m64; k += 16) {
+            double av[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[s] = P64b[o64 + static_cast<unsigned>(k + 4 * s)];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int kr = k + 4 * s + kq;
+                acc64[0] = Mma<double>::op(av[s], sPhi64[kr * PTS + col], acc64[0]);
+                acc64[1] = Mma<double>::op(av[s], sPhi64[(m64 + kr) * PTS + col], acc64[1]);
+            }
+        }
+        // the FP32 columns in chunks of 16, density loads two chunks ahead
+        // through a ring of three registers (as in k_rho)
+        const float* __restrict__ Bp = sPhi + (4 * kq) * PTS + col;
+        const int nch = mr >> 4;
+        auto chunk = [&](const v4f& av, int c) {
+            const float* b = Bp + c * 16 * PTS;
+            const float* e = b + mr * PTS;
+            const float b0 = b[0], b1 = b[PTS], b2 = b[2 * PTS], b3 = b[3 * PTS];
+            const float e0 = e[0], e1 = e[PTS], e2 = e[2 * PTS], e3 = e[3 * PTS];
+            __builtin_amdgcn_sched_barrier(0);  // the eight reads in flight together
+            acc[0] = Mma<float>::op(av[0], b0, acc[0]);
+            acc[1] = Mma<float>::op(av[0], e0, acc[1]);
+            acc[0] = Mma<float>::op(av[1], b1, acc[0]);
+            acc[1] = Mma<float>::op(av[1], e1, acc[1]);
+            acc[0] = Mma<float>::op(av[2], b2, acc[0]);
+            acc[1] = Mma<float>::op(av[2], e2, acc[1]);
+            acc[0] = Mma<float>::op(av[3], b3, acc[0]);
+            acc[1] = Mma<float>::op(av[3], e3, acc[1]);
+        };
+        v4f r2;
+        int c = 0;
+        for (; c + 3 <= nch; c += 3) {
+            r2 = load(c + 2);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(r0, c);
+            __builtin_amdgcn_sched_barrier(0);
+            r0 = load(c + 3);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(r1, c + 1);
+            __builtin_amdgcn_sched_barrier(0);
+            r1 = load(c + 4);
+            __builtin_amdgcn_sched_barrier(0);
+            chunk(r2, c + 2);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        if (c < nch) chunk(r0, c);
+        if (c + 1 < nch) chunk(r1, c + 1);
+    };
+    for (int it = wave; it < T16; it += 4) {
+        v4f acc[2];
+        v4d acc64[2];
+        if (it < L16) {
+            // leading row tile: contraction first, then its 32-byte slots
+            contract(it, acc, acc64);
+            double4 w[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    w[s][v] = s == 0 || two ? *reinterpret_cast<const double4*>(slot(it, v, s))
+                                            : make_double4(0.0, 0.0, 0.0, 0.0);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    accumulate(s, acc[s], acc64[s], v, AoVal{w[s][v].x, w[s][v].y, w[s][v].z, w[s][v].w});
+        } else {
+            contract(it, acc, acc64);
+            float4 w[2][4];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    w[s][v] = s == 0 || two ? *reinterpret_cast<const float4*>(slot(it, v, s))
+                                            : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    accumulate(s, acc[s], acc64[s], v, AoVal{w[s][v].x, w[s][v].y, w[s][v].z, w[s][v].w});
+        }
+    }
+    // Sum over the four row groups of the wave, then over the waves, in the
+    // order k_rho uses.
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sums[s][k] += __shfl_xor(sums[s][k], 16);
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) sums[s][k] += __shfl_xor(sums[s][k], 32);
+    if (kq == 0) {
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sRed[(wave * 4 + k) * PT2 + s * PTS + col] = sums[s][k];
+    }
+    __syncthreads();
+    if (tid < np_all) {
+        double r[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) r[k] = sRed[k * PT2 + tid];
+#pragma unroll
+        for (int w = 1; w < 4; ++w)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] += sRed[(w * 4 + k) * PT2 + tid];
+        const int g = blk.pt_begin + tile.y + tid;
+        const int go = a.rho_perm ? a.perm[g] : g;
+        a.rho_out[go] = make_double4(r[0], 2.0 * r[1], 2.0 * r[2], 2.0 * r[3]);
+    }
+}
+
 // Sums of e and n over the workgroup (kThreads threads) into *oe and *on, in
 // a fixed order: within each wave by xor shuffles, then the waves in order.
 __device__ __forceinline__ void wg_sum2(double e, double n, double* oe, double* on) {
@@ -1294,6 +1503,42 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
     return hipGetLastError();
 }
 
+// LDS of k_rho_aoc2: the sums, then the B operands of both sub-tiles.
+static size_t rho_aoc2_lds_bytes(const RhoClass& c) {
+    return sizeof(double) * 16 * 2 * kRhoPts + 2 * kRhoPts * (sizeof(double) * (size_t)c.max_m64 +
+                                                               sizeof(float) * (size_t)c.max_mpad);
+}
+
+// The mixed rho stage from the AO cache: k_rho_aoc2 where its LDS fits,
+// else k_rho<float, true, 1, kAocLoad> (one sub-tile at a time, less LDS).
+static hipError_t launch_rho_aoc2(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
+    const size_t lds = rho_aoc2_lds_bytes(c);
+    static int lim = 0;
+    if (lim == 0) {
+        int dev = 0, l = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho_aoc2),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, l);
+        if (e != hipSuccess) return e;
+        lim = l;
+    }
+    if (lds > static_cast<size_t>(lim)) return launch_rho_t<float, true, 1, kAocLoad>(a, c, s);
+    if (c.n_tiles <= 0) return hipSuccess;
+    RhoArgs b = a;
+    b.tiles = a.tiles + c.tile_begin;
+    b.aoc_off = a.aoc_off + c.tile_begin;
+    int longest = 0;
+    for (int x = 0; x < kXcds; ++x) {
+        b.seg[x] = c.seg[x];
+        longest = std::max(longest, c.seg[x + 1] - c.seg[x]);
+    }
+    b.seg[kXcds] = c.seg[kXcds];
+    hipLaunchKernelGGL(k_rho_aoc2, dim3(kXcds * longest), dim3(kThreads), lds, s, b);
+    return hipGetLastError();
+}
+
 size_t rho_lds_bytes(int mode, const RhoClass& c, int ns, bool cached) {
     const size_t ts = mode == kRhoMixed ? sizeof(float) : sizeof(double);
     const size_t staged = cached ? 0
@@ -1310,7 +1555,7 @@ hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t
     if (mode == kRhoFp64)
         return ns == 2 ? launch_rho_t<double, false, 2>(a, c, s) : launch_rho_t<double, false, 1>(a, c, s);
     if (a.aoc)
-        return ns == 2 ? launch_rho_t<float, true, 2, kAocLoad>(a, c, s) : launch_rho_t<float, true, 1, kAocLoad>(a, c, s);
+        return ns == 2 ? launch_rho_t<float, true, 2, kAocLoad>(a, c, s) : launch_rho_aoc2(a, c, s);
     return ns == 2 ? launch_rho_t<float, true, 2>(a, c, s) : launch_rho_t<float, true, 1>(a, c, s);
 }
 
