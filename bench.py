@@ -169,6 +169,108 @@ def load_valu(workload: str):
     return t
 
 
+# --------------------------------------------------------------------------
+# Digests of rocprofv3 counter runs into the summaries read above
+# --------------------------------------------------------------------------
+# Kernels of each rho stage (single spin) and of the VALU-roofline stages, by
+# rocprofv3 kernel name.  The mixed rho stage from the AO cache is
+# k_rho_aoc2 (k_rho<float, true, 1, 2> for a launch class whose LDS does not
+# fit it).
+RHO_STAGE_KERNELS = {"fp64": ("k_rho<double, false, 1, 0>",),
+                     "fp32": ("k_rho_aoc2", "k_rho<float, true, 1, 2>")}
+VALU_STAGES = {
+    "ao_evaluation_fp64_rho": ("rho", "fp64", ("k_rho<double, false, 1, 0>",)),
+    "functional": ("xc", "fp64", ("k_xc<3>",)),
+    "ao_evaluation_fp32_v": ("vxc", "fp32", ("k_vxc_group_pipe<17>", "k_vxc_group_pipe<8>")),
+    "v_fp64": ("vxc", "fp64", ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>")),
+    "rho_fp32_from_cache": ("rho", "fp32", ("k_rho_aoc2", "k_rho<float, true, 1, 2>")),
+}
+
+
+def _pmc_rows(run_dir: str):
+    import csv
+    path = os.path.join(run_dir, "pass_counter_collection.csv")
+    with open(path) as f:
+        return list(csv.DictReader(f))
+
+
+def _kernel_of(name: str, wanted) -> str:
+    """The entry of `wanted` that names this dispatch's kernel (matched on the template-qualified name)."""
+    base = name.split("(")[0].replace("void ", "").replace("chx::", "").strip()
+    return base if base in wanted else ""
+
+
+def _per_stage(rows, counter: str, kernels):
+    """(counter total, dispatches, stages) over the dispatches of `kernels`; a stage is one launch per
+    class, the classes told apart by grid size."""
+    tot, n, grids = 0.0, 0, set()
+    for r in rows:
+        if r["Counter_Name"] == counter and _kernel_of(r["Kernel_Name"], kernels):
+            tot += float(r["Counter_Value"])
+            n += 1
+            grids.add(r["Grid_Size"])
+    if grids and n % len(grids):
+        raise ValueError(f"{counter}: {n} dispatches of {kernels} do not split into {len(grids)} classes")
+    stages = n // max(len(grids), 1)
+    return tot, n, stages
+
+
+def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str) -> None:
+    """profiles/k_rho_<workload>_<prec>_traffic.json from one FETCH_SIZE and one WRITE_SIZE pass, each
+    scaled by its reading for the 1 GiB device-to-device copy of the same run."""
+    fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
+    copy = 1 << 30
+
+    def copy_kib(rows, counter):
+        vals = [float(r["Counter_Value"]) for r in rows
+                if r["Counter_Name"] == counter and "copyBuffer" in r["Kernel_Name"]]
+        return max(vals)  # the calibration copy is the largest device-to-device copy of the run
+
+    fk, wk = copy_kib(fr, "FETCH_SIZE"), copy_kib(wr, "WRITE_SIZE")
+    f_scale, w_scale = copy / (fk * 1024.0), copy / (wk * 1024.0)
+    for prec, kernels in RHO_STAGE_KERNELS.items():
+        ft, fn, fs = _per_stage(fr, "FETCH_SIZE", kernels)
+        wt, wn, ws = _per_stage(wr, "WRITE_SIZE", kernels)
+        if fs == 0 or ws == 0:
+            continue
+        out = {"workload": workload, "precision": prec, "kernel": "k_rho", "sources_sha256": sources_digest(),
+               "kernels": list(kernels), "stages": {"fetch": fs, "write": ws},
+               "fetch_kib_per_build_raw": ft / fs, "write_kib_per_build_raw": wt / ws,
+               "copy_calibration": {"bytes": copy, "fetch_kib": fk, "write_kib": wk, "fetch_scale": f_scale,
+                                    "write_scale": w_scale},
+               "hbm_bytes_per_build": ft / fs * 1024.0 * f_scale + wt / ws * 1024.0 * w_scale,
+               "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one pass each) over {cmd}; sum over the "
+                         "single-spin rho-stage launches of one build (run total / stages), each counter scaled "
+                         "by (1 GiB / its reading for the 1 GiB device-to-device copy kernel)"}
+        path = os.path.join(ROOT, "profiles", f"k_rho_{workload}_{prec}_traffic.json")
+        with open(path, "w") as f:
+            json.dump(out, f, indent=1)
+        log(f"wrote {path}: {out['hbm_bytes_per_build'] / 1e6:.1f} MB per build")
+
+
+def digest_valu(sq_dir: str, workload: str, cmd: str) -> None:
+    """profiles/valu_<workload>.json: VALU (SQ_INSTS_VALU - SQ_INSTS_MFMA) instructions per build of each stage."""
+    rows = _pmc_rows(sq_dir)
+    stages = {}
+    for name, (stage, prec, kernels) in VALU_STAGES.items():
+        v, _, n_v = _per_stage(rows, "SQ_INSTS_VALU", kernels)
+        m, _, n_m = _per_stage(rows, "SQ_INSTS_MFMA", kernels)
+        if n_v == 0:
+            continue
+        # the V stage runs one launch per group class in every build: stages = launches of one class
+        stages[name] = {"stage": stage, "precision": prec, "kernels": list(kernels),
+                        "valu_instr_per_build": (v - m) / n_v, "mfma_instr_per_build": m / n_v, "stages": n_v}
+    out = {"workload": workload, "sources_sha256": sources_digest(),
+           "source": f"rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_MFMA SQ_INSTS_LDS SQ_INSTS_SALU "
+                     f"SQ_INSTS_VMEM_RD SQ_INSTS_SMEM SQ_WAVE_CYCLES over {cmd}; VALU = SQ_INSTS_VALU - "
+                     "SQ_INSTS_MFMA per build (run total / stages)",
+           "stages": stages}
+    path = os.path.join(ROOT, "profiles", f"valu_{workload}.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    log(f"wrote {path}: " + ", ".join(f"{k} {e['valu_instr_per_build']:.3e}" for k, e in stages.items()))
+
+
 def valu_rooflines(workload: str, kms_by_prec: dict):
     """VALU roofline of the AO-evaluation and functional stages: counted instructions per build over the
     stage time measured in this run, against the full-rate VALU issue of the chip."""
@@ -370,7 +472,19 @@ def main() -> int:
                     help="mixed build's AO cache: 0 auto, 1 off, 2 on")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
+    ap.add_argument("--digest-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
+                    help="write profiles/k_rho_<workload>_<prec>_traffic.json from two rocprofv3 --pmc runs "
+                         "(FETCH_SIZE, WRITE_SIZE) of this script with --copy-calibration-mib 1024, and exit")
+    ap.add_argument("--digest-valu", metavar="SQ_DIR", default=None,
+                    help="write profiles/valu_<workload>.json from a rocprofv3 --pmc run of SQ counters, and exit")
+    ap.add_argument("--digest-cmd", default="bench.py", help="the profiled command, recorded in the digest")
     args = ap.parse_args()
+    if args.digest_traffic or args.digest_valu:
+        if args.digest_traffic:
+            digest_traffic(*args.digest_traffic, args.workload, args.digest_cmd)
+        if args.digest_valu:
+            digest_valu(args.digest_valu, args.workload, args.digest_cmd)
+        return 0
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): the Vxc build runs on one "
