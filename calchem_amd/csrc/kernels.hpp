@@ -32,15 +32,25 @@ struct RhoArgs {
 };
 
 // AO cache of the mixed rho stage: per rho workgroup tile, per 16-point
-// sub-tile, per 16-row tile of the block, the (phi, dphi/dx, dphi/dy,
-// dphi/dz) of the 64 lanes x 4 slots exactly as the rho kernel's lanes hold
-// them (slot v of lane (j, q): row Mma<float>::row(q, v) of the row tile,
-// point j): [v][lane] of double4 for the block's leading row tiles (rows kept
-// FP64), of float4 for the others.
-constexpr long long kAocRowTile64 = 64LL * 4 * 32;  // bytes per leading row tile
-constexpr long long kAocRowTile32 = 64LL * 4 * 16;  // bytes per FP32 row tile
+// sub-tile, two planes over the block's 16-row tiles in order: phi, then
+// (dphi/dx, dphi/dy, dphi/dz), each [v][lane] exactly as the rho kernel's
+// lanes hold them (slot v of lane (j, q): row Mma<float>::row(q, v) of the
+// row tile, point j), FP64 for the block's leading row tiles (rows kept FP64)
+// and FP32 for the others.  The kernels read phi alone to stage the MFMA
+// operand and the gradients (with phi) for the epilogue, so the first read
+// touches a quarter of the bytes.
+constexpr long long kAocPhi64 = 64LL * 4 * 8;    // bytes of one leading row tile's phi plane
+constexpr long long kAocPhi32 = 64LL * 4 * 4;    // ... of an FP32 row tile
+constexpr long long kAocGrad64 = 64LL * 4 * 24;  // bytes of one leading row tile's gradient plane
+constexpr long long kAocGrad32 = 64LL * 4 * 12;  // ... of an FP32 row tile
+__host__ __device__ inline long long aoc_phi_off(int it, int l16) {
+    return it < l16 ? it * kAocPhi64 : l16 * kAocPhi64 + (it - l16) * kAocPhi32;
+}
+__host__ __device__ inline long long aoc_grad_off(int it, int l16, int t16) {
+    return aoc_phi_off(t16, l16) + (it < l16 ? it * kAocGrad64 : l16 * kAocGrad64 + (it - l16) * kAocGrad32);
+}
 __host__ __device__ inline long long aoc_sub_bytes(int m_pad, int m64) {
-    return (long long)(m64 >> 4) * kAocRowTile64 + (long long)((m_pad - m64) >> 4) * kAocRowTile32;
+    return aoc_grad_off(m_pad >> 4, m64 >> 4, m_pad >> 4);
 }
 
 // Functional-stage workgroups never straddle molecules: workgroup w covers

@@ -201,40 +201,49 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         const int np = min(PTS, np_all - sub * PTS), tsub = t0 + sub * PTS;
         if (sub > 0) preload();
 
-        // Slot v of row tile it of this sub-tile in the AO cache.
+        // Slot v of row tile it of this sub-tile in the AO cache: its phi
+        // and its gradient (see aoc_sub_bytes).
         unsigned char* const csub = AOC != kAocNone ? cache_base + sub * sub_bytes : nullptr;
-        auto cslot = [&](int it, int v) {
-            return it < L16 ? csub + it * kAocRowTile64 + (v * 64 + lane) * 32
-                            : csub + L16 * kAocRowTile64 + (it - L16) * kAocRowTile32 + (v * 64 + lane) * 16;
+        auto cphi = [&](int it, int v) {
+            return csub + aoc_phi_off(it, L16) + (v * 64 + lane) * (it < L16 ? 8 : 4);
         };
-        // The four slots of row tile it: raw words first, so that several
-        // row tiles' loads are in flight before any is used.  Both halves are
-        // loaded unconditionally (a branch here makes every load wait): a
-        // leading row tile's slot fills both, an FP32 slot the first and
-        // reads itself again into the second, which stays unused.
-        auto craw = [&](int it, uint4 (&w)[4][2]) {
-            const int second = it < L16 ? 1 : 0;
+        auto cgrad = [&](int it, int v) {
+            return csub + aoc_grad_off(it, L16, T16) + (v * 64 + lane) * (it < L16 ? 24 : 12);
+        };
+        // (phi, grad phi) of the four slots of row tile it
+        auto cload = [&](int it, AoVal (&f)[4]) {
+            if (it < L16) {
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const uint4* p = reinterpret_cast<const uint4*>(cslot(it, v));
-                w[v][0] = p[0];
-                w[v][1] = p[second];
+                for (int v = 0; v < 4; ++v) {
+                    const double* g = reinterpret_cast<const double*>(cgrad(it, v));
+                    f[v] = AoVal{*reinterpret_cast<const double*>(cphi(it, v)), g[0], g[1], g[2]};
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float* g = reinterpret_cast<const float*>(cgrad(it, v));
+                    f[v] = AoVal{*reinterpret_cast<const float*>(cphi(it, v)), g[0], g[1], g[2]};
+                }
             }
         };
-        auto cunpack = [&](int it, const uint4 (&w)[2]) {
-            if (it < L16)
-                return AoVal{__hiloint2double(w[0].y, w[0].x), __hiloint2double(w[0].w, w[0].z),
-                             __hiloint2double(w[1].y, w[1].x), __hiloint2double(w[1].w, w[1].z)};
-            return AoVal{__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z),
-                         __uint_as_float(w[0].w)};
+        auto cload_phi = [&](int it, int v) {
+            return it < L16 ? *reinterpret_cast<const double*>(cphi(it, v))
+                            : static_cast<double>(*reinterpret_cast<const float*>(cphi(it, v)));
         };
         auto cstore = [&](int it, int v, const AoVal& f) {
-            if (it < L16)
-                *reinterpret_cast<double4*>(cslot(it, v)) = make_double4(f.v, f.gx, f.gy, f.gz);
-            else
-                *reinterpret_cast<float4*>(cslot(it, v)) =
-                    make_float4(static_cast<float>(f.v), static_cast<float>(f.gx), static_cast<float>(f.gy),
-                                static_cast<float>(f.gz));
+            if (it < L16) {
+                *reinterpret_cast<double*>(cphi(it, v)) = f.v;
+                double* g = reinterpret_cast<double*>(cgrad(it, v));
+                g[0] = f.gx;
+                g[1] = f.gy;
+                g[2] = f.gz;
+            } else {
+                *reinterpret_cast<float*>(cphi(it, v)) = static_cast<float>(f.v);
+                float* g = reinterpret_cast<float*>(cgrad(it, v));
+                g[0] = static_cast<float>(f.gx);
+                g[1] = static_cast<float>(f.gy);
+                g[2] = static_cast<float>(f.gz);
+            }
         };
 
         // Radial sums once per (shell, point).
@@ -273,16 +282,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         }
         AoVal cache[kRhoCached][4];
         if constexpr (AOC == kAocLoad) {
-            // (a wave without a j-th row tile loads the block's last one
-            // again: unconditional loads, none of them waited on early)
-            uint4 w[kRhoCached][4][2];
-#pragma unroll
-            for (int j = 0; j < kRhoCached; ++j) craw(min(wave + 4 * j, T16 - 1), w[j]);
 #pragma unroll
             for (int j = 0; j < kRhoCached; ++j)
-                if (wave + 4 * j < T16)
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) cache[j][v] = cunpack(wave + 4 * j, w[j][v]);
+                if (wave + 4 * j < T16) cload(wave + 4 * j, cache[j]);
         }
 #pragma unroll
         for (int j = 0; j < kRhoCached; ++j) {
@@ -298,12 +300,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
             }
         }
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
-            uint4 w[4][2];
-            if constexpr (AOC == kAocLoad) craw(it, w);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int r = (it << 4) + M::row(kq, v);
-                const double fv = AOC == kAocLoad ? cunpack(it, w[v]).v : eval(it, v).v;
+                const double fv = AOC == kAocLoad ? cload_phi(it, v) : eval(it, v).v;
                 sPhi[r * LDP + col] = static_cast<TS>(fv);
                 if (HYB && r < m64) sPhi64[r * LDP + col] = fv;
             }
@@ -411,13 +411,13 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
         for (int it = wave + 4 * kRhoCached; it < T16; it += 4) {
             typename M::acc acc[NS];
             v4d acc64[NS];
-            uint4 w[4][2];
-            if constexpr (AOC == kAocLoad) craw(it, w);
+            AoVal w[4];
+            if constexpr (AOC == kAocLoad) cload(it, w);
 #pragma unroll
             for (int sp = 0; sp < NS; ++sp) contract(it, acc[sp], acc64[sp], false, sp);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const AoVal f = AOC == kAocLoad ? cunpack(it, w[v]) : eval(it, v);
+                const AoVal f = AOC == kAocLoad ? w[v] : eval(it, v);
 #pragma unroll
                 for (int sp = 0; sp < NS; ++sp) accumulate(sp, acc[sp], acc64[sp], v, f);
             }
@@ -495,15 +495,20 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     // the block's density blocks: uniform bases, 32-bit per-lane offsets
     const float* __restrict__ PSb = static_cast<const float*>(a.PS) + blk.psub;
     const double* __restrict__ P64b = a.P64 + blk.psub;
-    // Slot v of row tile it of sub-tile s in the AO cache (see aoc_sub_bytes).
+    // Slot v of row tile it of sub-tile s in the AO cache: its phi and its
+    // gradient (see aoc_sub_bytes).  A tile without a second sub-tile reads
+    // the first one in its place: that sub-tile's sums are then never written.
     const unsigned char* const cbase = static_cast<const unsigned char*>(a.aoc) + (a.aoc_off[t] << 4);
-    const long long sub_bytes = aoc_sub_bytes(mp, m64);
-    auto slot = [&](int it, int v, int s) {
-        const unsigned char* c = cbase + s * sub_bytes;
-        return it < L16 ? c + it * kAocRowTile64 + static_cast<unsigned>((v * 64 + lane) * 32)
-                        : c + L16 * kAocRowTile64 + (it - L16) * kAocRowTile32 + static_cast<unsigned>((v * 64 + lane) * 16);
+    const long long sub1 = two ? aoc_sub_bytes(mp, m64) : 0;
+    auto phi_at = [&](int it, int v, int s) {
+        return cbase + (s ? sub1 : 0) + aoc_phi_off(it, L16) +
+               static_cast<unsigned>((v * 64 + lane) * (it < L16 ? 8 : 4));
     };
-    // B operands: phi of both sub-tiles (a missing second sub-tile is zero).
+    auto grad_at = [&](int it, int v, int s) {
+        return cbase + (s ? sub1 : 0) + aoc_grad_off(it, L16, T16) +
+               static_cast<unsigned>((v * 64 + lane) * (it < L16 ? 24 : 12));
+    };
+    // B operands: phi of both sub-tiles.
     for (int it = wave; it < T16; it += 4) {
         if (it < L16) {
             double f[2][4];
@@ -511,7 +516,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    f[s][v] = s == 0 || two ? *reinterpret_cast<const double*>(slot(it, v, s)) : 0.0;
+                    f[s][v] = *reinterpret_cast<const double*>(phi_at(it, v, s));
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -522,7 +527,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
             for (int s = 0; s < 2; ++s)
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    f[s][v] = s == 0 || two ? *reinterpret_cast<const float*>(slot(it, v, s)) : 0.f;
+                    f[s][v] = *reinterpret_cast<const float*>(phi_at(it, v, s));
 #pragma unroll
             for (int s = 0; s < 2; ++s)
 #pragma unroll
@@ -614,32 +619,33 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
         if (it < L16) {
             // leading row tile: contraction first, then its 32-byte slots
             contract(it, acc, acc64);
-            double4 w[2][4];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s) {
+                AoVal w[4];
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    w[s][v] = s == 0 || two ? *reinterpret_cast<const double4*>(slot(it, v, s))
-                                            : make_double4(0.0, 0.0, 0.0, 0.0);
+                for (int v = 0; v < 4; ++v) {
+                    const double* g = reinterpret_cast<const double*>(grad_at(it, v, s));
+                    w[v] = AoVal{*reinterpret_cast<const double*>(phi_at(it, v, s)), g[0], g[1], g[2]};
+                }
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    accumulate(s, acc[s], acc64[s], v, AoVal{w[s][v].x, w[s][v].y, w[s][v].z, w[s][v].w});
+                for (int v = 0; v < 4; ++v) accumulate(s, acc[s], acc64[s], v, w[v]);
+                __builtin_amdgcn_sched_barrier(0);
+            }
         } else {
             contract(it, acc, acc64);
-            float4 w[2][4];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int s = 0; s < 2; ++s) {
+                float4 w[4];
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const float* g = reinterpret_cast<const float*>(grad_at(it, v, s));
+                    w[v] = make_float4(*reinterpret_cast<const float*>(phi_at(it, v, s)), g[0], g[1], g[2]);
+                }
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
-                    w[s][v] = s == 0 || two ? *reinterpret_cast<const float4*>(slot(it, v, s))
-                                            : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    accumulate(s, acc[s], acc64[s], v, AoVal{w[s][v].x, w[s][v].y, w[s][v].z, w[s][v].w});
+                    accumulate(s, acc[s], acc64[s], v, AoVal{w[v].x, w[v].y, w[v].z, w[v].w});
+                __builtin_amdgcn_sched_barrier(0);
+            }
         }
     }
     // Sum over the four row groups of the wave, then over the waves, in the
