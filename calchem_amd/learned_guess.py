@@ -574,9 +574,53 @@ def evaluate(model_path: str, n_clusters: int, seed: int, n_waters: int = 16, po
     return rows
 
 
+def diagnose(model_path: str, n_clusters: int, seed: int, n_waters: int = 16) -> dict:
+    """Which pair types' errors cost SCF iterations: starts built from the
+    learned blocks with the blocks of some pair types replaced by those of
+    the converged density (FP64 SCF from SAD on the same solver), each
+    purified as the default guess is, against SAD."""
+    from .grid import MolecularGrid
+    from .scf import RksSolver, ScfOptions
+    model = LearnedGuess.load(model_path)
+    cases = {"learned": lambda t: np.zeros(len(t), bool), "exact_onsite": lambda t: t < 2,
+             "exact_intra": lambda t: t < 4, "exact_inter": lambda t: t >= 4,
+             "exact_all": lambda t: np.ones(len(t), bool)}
+    its = {"sad": [], **{k: [] for k in cases}}
+    for k in range(n_clusters):
+        mol = perturbed_cluster(n_waters, 100000 * seed + k)
+        basis = BasisSet.from_name(BASIS_NAME, mol)
+        grid = MolecularGrid.build_for_basis(mol, basis, "medium")
+        s = RksSolver(mol, basis, XC, grid, ScfOptions(cholesky_tau=1e-8))
+        r0 = s.run()
+        its["sad"].append(r0.iterations)
+        W = water_topology(mol)
+        pairs = pair_list(mol, W)
+        Y = model.predict(pair_features(mol, W, pairs), pairs[:, 2])
+        Yx, _ = local_blocks(mol, basis, W, pairs, r0.density)
+        for name, sel in cases.items():
+            P = assemble_density(mol, basis, W, pairs, np.where(sel(pairs[:, 2])[:, None], Yx, Y))
+            for _ in range(2):
+                D = 0.5 * P
+                DS = D @ s.S
+                D = 3.0 * DS @ D - 2.0 * DS @ DS @ D
+                P = D + D.T
+            P = P * (2.0 * s.n_occ / float(np.sum(P * s.S)))
+            its[name].append(s.run(guess_density=P, guess_name=name).iterations)
+        s.close()
+        _log(f"cluster {k}: " + ", ".join(f"{a} {b[-1]}" for a, b in its.items()))
+    model.close()
+    return {"clusters": n_clusters, "waters": n_waters, "seed": seed,
+            "mean_iterations": {a: float(np.mean(b)) for a, b in its.items()}, "iterations": its}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description="train and evaluate the learned SCF guess on the GPU")
     sub = ap.add_subparsers(dest="cmd", required=True)
+    dg = sub.add_parser("diagnose", help="SCF iterations from learned starts with some pair types' blocks exact")
+    dg.add_argument("--model", default=DEFAULT_MODEL)
+    dg.add_argument("--clusters", type=int, default=8)
+    dg.add_argument("--seed", type=int, default=1004)
+    dg.add_argument("--report", default=None)
     tr = sub.add_parser("train")
     tr.add_argument("--n-train", type=int, default=2000)
     tr.add_argument("--n-eval", type=int, default=100)
@@ -588,6 +632,13 @@ def main(argv=None):
     tr.add_argument("--out", default=DEFAULT_MODEL)
     tr.add_argument("--report", default=None)
     a = ap.parse_args(argv)
+    if a.cmd == "diagnose":
+        rep = diagnose(a.model, a.clusters, a.seed)
+        _log("mean iterations: " + ", ".join(f"{k} {v:.3f}" for k, v in rep["mean_iterations"].items()))
+        if a.report:
+            with open(a.report, "w") as fh:
+                json.dump(rep, fh, indent=1)
+        return 0
     t0 = time.time()
     # the worker processes start before this process touches the GPU and serve
     # both the data generation and the evaluation

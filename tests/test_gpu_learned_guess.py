@@ -41,3 +41,13 @@ def test_model_declines_what_it_does_not_cover(model):
     assert m.density(mol, basis, np.eye(basis.n_ao), 9) is None
     w = LG.perturbed_cluster(3, 5)
     assert m.density(w, BasisSet.from_name("6-31g(d,p)", w), np.eye(75), 15) is None
+
+
+def test_diagnosis_orders_the_starts(model):
+    # exact blocks everywhere start next to the converged density; the
+    # learned start sits between that and SAD
+    rep = LG.diagnose(model, 1, 1004, n_waters=8)
+    it = {k: v[0] for k, v in rep["iterations"].items()}
+    print("iterations:", it)
+    assert it["exact_all"] < it["learned"] < it["sad"]
+    assert it["exact_intra"] <= it["learned"] + 1
