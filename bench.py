@@ -443,7 +443,7 @@ def bench_batched(args) -> int:
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if best == "fp32" else "f64", "data": "synthetic",
         "config": {"workload": f"batch{len(systems)}: {len(systems)} water clusters (2-32 waters, seed 3), "
-                               f"6-31G(d) ({min(n_ao)}-{max(n_ao)} AO), 50x194 Becke grid ({n_pts} points), "
+                               f"6-31G(d,p) ({min(n_ao)}-{max(n_ao)} AO), 50x194 Becke grid ({n_pts} points), "
                                f"{func.upper()}", "precision": best},
         "modes": out,
     }
