@@ -246,14 +246,14 @@ struct Rad3 {
 };
 
 __device__ __forceinline__ Rad3 shell_radial3(const DevShell& s, const double2* __restrict__ prims, double x,
-                                              double y, double z) {
+                                              double y, double z, const double* __restrict__ tab) {
     const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
     const double r2 = dx * dx + dy * dy + dz * dz;
     Rad3 o{0.0, 0.0, 0.0};
     if (r2 <= s.r2cut) {
         for (int p = 0; p < s.n_prims; ++p) {
             const double2 pr = prims[s.prim_start + p];
-            const double e = pr.y * exp(-pr.x * r2);
+            const double e = pr.y * exp_neg_tab(pr.x * r2, tab);
             o.r0 += e;
             o.r1 -= 2.0 * pr.x * e;
             o.r2 += 4.0 * pr.x * pr.x * e;
@@ -297,6 +297,22 @@ __device__ __forceinline__ AoHess ao_hess_row(const DevShell& s, const RowL& r, 
     o.h[4] = sc * (px * dpy * dpz * rr.r0 + (dm1 * dz + dm2 * dy) * rr.r1 + m * dy * dz * rr.r2);
     o.h[5] = sc * (px * py * d2pz * rr.r0 + (2.0 * dm2 * dz + m) * rr.r1 + m * dz * dz * rr.r2);
     return o;
+}
+
+// Value and gradient of one row from (R0, R1) of its shell, without the
+// per-AO drop rule (the gradient's AO values, eval_aos_hess, have none):
+// the v and g of ao_hess_row.
+__device__ __forceinline__ AoVal ao_grad_row(const DevShell& s, const RowL& r, double x, double y, double z,
+                                             const Rad3& rr) {
+    const double dx = x - s.cx, dy = y - s.cy, dz = z - s.cz;
+    double px, dpx, d2px, py, dpy, d2py, pz, dpz, d2pz;
+    mono_d2(dx, r.lmn & 3, px, dpx, d2px);
+    mono_d2(dy, (r.lmn >> 2) & 3, py, dpy, d2py);
+    mono_d2(dz, (r.lmn >> 4) & 3, pz, dpz, d2pz);
+    const double m = px * py * pz, sc = r.scale;
+    const double dm0 = dpx * py * pz, dm1 = px * dpy * pz, dm2 = px * py * dpz;
+    return AoVal{sc * m * rr.r0, sc * (dm0 * rr.r0 + m * dx * rr.r1), sc * (dm1 * rr.r0 + m * dy * rr.r1),
+                 sc * (dm2 * rr.r0 + m * dz * rr.r1)};
 }
 
 // Stage the rows, shells (with block-local primitive offsets) and primitives

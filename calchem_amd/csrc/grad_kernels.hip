@@ -66,7 +66,10 @@ __global__ __launch_bounds__(kThreads) void k_xc_grad(XcGradArgs a) {
 // AO-force stage for one spin (k_rho's tiling: one workgroup per 16 points of
 // a block).  LDS: points, (w v, w u) per point, host per point, radial
 // (R0, R1, R2) per (shell, point), shells, primitives, rows, shell atoms, Phi
-// and Y [row][point], per-row force sums, per-wave per-point force sums.
+// and Y [row][point], per-row force sums, the exponential's table, per-wave
+// per-point force sums.  The staging pass forms phi and u . grad phi from
+// values and gradients alone; the epilogue evaluates the Hessians of the
+// wave's rows.
 __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PTS = kRhoPts;
@@ -98,9 +101,11 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
     double* sY = reinterpret_cast<double*>(q);  // [mp][PTS]
     q += sizeof(double) * mp * PTS;
     double* sRowF = reinterpret_cast<double*>(q);  // [mp][3]
+    double* sExp = sRowF + 3 * mp;                 // [64] the exponential's table
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int col = lane & 15, kq = lane >> 4;
+    if (tid < 64) sExp[tid] = kExpNegTab[tid];
     stage_meta(a.block_rows + blk.ao_begin, m, mp, a.block_sh + blk.sh_begin, nsh, a.shells, a.prims, a.aos, sRow,
                sSh, sPrim, tid, kThreads);
     for (int s = tid; s < nsh; s += kThreads) sShAtom[s] = a.shell_atom[a.block_sh[blk.sh_begin + s].x];
@@ -119,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
         Rad3 rr{0.0, 0.0, 0.0};
         if (p < np) {
             const double4 x = sPt[p];
-            rr = shell_radial3(sSh[sh], sPrim, x.x, x.y, x.z);
+            rr = shell_radial3(sSh[sh], sPrim, x.x, x.y, x.z, sExp);
         }
         sRad[idx] = rr;
     }
@@ -130,10 +135,10 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
         double v = 0.0, y = 0.0;
         if (row.sh >= 0 && p < np) {
             const double4 x = sPt[p];
-            const AoHess f = ao_hess_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + p]);
+            const AoVal f = ao_grad_row(sSh[row.sh], row, x.x, x.y, x.z, sRad[row.sh * PTS + p]);
             const double4 w = sW[p];
             v = f.v;
-            y = w.y * f.g[0] + w.z * f.g[1] + w.w * f.g[2];
+            y = w.y * f.gx + w.z * f.gy + w.w * f.gz;
         }
         sPhi[r * PTS + p] = v;
         sY[r * PTS + p] = y;
@@ -145,11 +150,17 @@ __global__ __launch_bounds__(kThreads) void k_grad_ao(GradAoArgs a) {
         const int i0 = it << 4;
         v4d accS = {0.0, 0.0, 0.0, 0.0}, accT = accS;
         const double* __restrict__ Arow = a.PS + blk.psub + static_cast<long long>(i0 + col) * mp;
-        for (int k = 0; k < mp; k += 4) {
-            const int kk = k + kq;
-            const double av = Arow[kk];
-            accS = mfma64(av, sPhi[kk * PTS + col], accS);
-            accT = mfma64(av, sY[kk * PTS + col], accT);
+        // four k-steps per round, their density loads issued together (m_pad is a multiple of 16)
+        for (int k = 0; k < mp; k += 16) {
+            double av[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) av[s] = Arow[k + 4 * s + kq];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) {
+                const int kk = k + 4 * s + kq;
+                accS = mfma64(av[s], sPhi[kk * PTS + col], accS);
+                accT = mfma64(av[s], sY[kk * PTS + col], accT);
+            }
         }
         // Element v of this lane: row i0 + 4 v + kq (the f64 result layout), point col.
 #pragma unroll
@@ -502,7 +513,7 @@ size_t grad_ao_lds_bytes(int max_mpad, int max_nsh, int max_prim) {
            sizeof(Rad3) * (size_t)((max_nsh * kRhoPts + 1) & ~1) + sizeof(DevShell) * (size_t)max_nsh +
            sizeof(double2) * (size_t)max_prim + sizeof(RowL) * (size_t)max_mpad +
            sizeof(int) * (size_t)((max_nsh + 3) & ~3) + sizeof(double) * 2 * (size_t)max_mpad * kRhoPts +
-           sizeof(double) * 3 * (size_t)max_mpad;
+           sizeof(double) * 3 * (size_t)max_mpad + sizeof(double) * 64;
 }
 
 hipError_t launch_xc_grad(const XcGradArgs& a, hipStream_t s) {
