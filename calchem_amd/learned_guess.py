@@ -45,7 +45,8 @@ from .basis import CART_COMPONENTS, BasisSet, primitive_norm
 from .configs import SplitMix64, water_cluster
 from .molecule import ANGSTROM_TO_BOHR, Atom, Molecule
 
-D_IN, HIDDEN, D_OUT = 80, 128, 225
+D_IN, HIDDEN, D_OUT = 132, 128, 225
+D_IN_BASE = 80  # the pair features; D_IN adds the environment of both waters (env_features)
 TYPES = ["onsite_O", "onsite_H", "intra_OH", "intra_HH", "inter_OO", "inter_OH", "inter_HO", "inter_HH"]
 CUTOFF_ANGSTROM = 7.0          # O-O distance below which the waters' pair blocks are predicted
 CHARGES = {8: -0.834, 1: 0.417}  # point charges (e) of the field features
@@ -184,6 +185,56 @@ def pair_list(mol: Molecule, W: np.ndarray) -> np.ndarray:
     return np.array(rows, dtype=np.int64).reshape(-1, 5)
 
 
+def _env_terms(points: np.ndarray, src: np.ndarray, q: np.ndarray):
+    """Potential, field and field gradient at points from point charges q at src."""
+    d = points[:, None, :] - src[None, :, :]
+    r = np.linalg.norm(d, axis=-1)
+    r = np.where(r < 1e-6, np.inf, r)
+    pot = np.sum(q[None, :] / r, axis=1)
+    E = np.sum(q[None, :, None] * d / r[..., None] ** 3, axis=1)
+    G = np.einsum("k,pka,pkb->pab", q, d, d * (3.0 / r[..., None] ** 5)) - \
+        np.eye(3)[None] * np.sum(q[None, :] / r ** 3, axis=1)[:, None, None]
+    return pot, E, G
+
+
+def env_features(mol: Molecule, W: np.ndarray, pairs: np.ndarray) -> np.ndarray:
+    """The environment of the pair's two waters i and j (52 per pair), from the
+    point charges of the other waters, in the frame of water i: electric
+    fields at the three atoms of i and of j (18), potentials there (6), the
+    field gradient at both oxygens (xx, yy, xy, xz, yz: 10), and Gaussian
+    neighbour counts by element at three widths around the three atoms of i
+    (18)."""
+    R = mol.coords_bohr() / ANGSTROM_TO_BOHR  # Angstrom
+    F = water_frames(R, W)
+    z = mol.z_array()
+    q = np.array([CHARGES[int(v)] for v in z])
+    water_of = np.repeat(np.arange(len(W)), 3)
+    out = np.zeros((len(pairs), D_IN - D_IN_BASE))
+    cache = {}
+    for k, (_, _, _, i, j) in enumerate(pairs):
+        if (i, j) not in cache:
+            v = np.zeros(D_IN - D_IN_BASE)
+            other = (water_of != i) & (water_of != j)
+            if other.any():
+                Fi = F[i]
+                src, qs = R[other], q[other]
+                pot, E, G = _env_terms(np.concatenate([R[W[i]], R[W[j]]]), src, qs)
+                v[0:18] = (E @ Fi).ravel()
+                v[18:24] = pot
+                for s_, p_ in enumerate((0, 3)):
+                    Gl = Fi.T @ G[p_] @ Fi
+                    v[24 + 5 * s_:29 + 5 * s_] = [Gl[0, 0], Gl[1, 1], Gl[0, 1], Gl[0, 2], Gl[1, 2]]
+                zs = z[other]
+                for s_, c in enumerate(W[i]):
+                    d = np.linalg.norm(src - R[c], axis=1)
+                    for e, el in enumerate((8, 1)):
+                        for w, sig in enumerate(ENV_WIDTHS):
+                            v[34 + 6 * s_ + 3 * e + w] = np.sum(np.exp(-(d[zs == el] / sig) ** 2))
+            cache[(i, j)] = v
+        out[k] = cache[(i, j)]
+    return out
+
+
 def pair_features(mol: Molecule, W: np.ndarray, pairs: np.ndarray) -> np.ndarray:
     R = mol.coords_bohr() / ANGSTROM_TO_BOHR  # Angstrom
     F = water_frames(R, W)
@@ -192,6 +243,7 @@ def pair_features(mol: Molecule, W: np.ndarray, pairs: np.ndarray) -> np.ndarray
     water_of = np.repeat(np.arange(len(W)), 3)
     role = np.tile(np.arange(3), len(W))
     X = np.zeros((len(pairs), D_IN), dtype=np.float64)
+    X[:, D_IN_BASE:] = env_features(mol, W, pairs)
     for k, (a, b, t, i, j) in enumerate(pairs):
         Fi = F[i]
         oi = R[W[i, 0]]
@@ -346,6 +398,11 @@ class LearnedGuess:
     def __init__(self, params, x_mean, x_std, y_mean, y_std, meta=None):
         self.params = np.asarray(params, dtype=np.float32).reshape(len(TYPES), -1)
         self.x_mean, self.x_std = np.asarray(x_mean), np.asarray(x_std)
+        # inputs of the networks: D_IN, or the D_IN_BASE pair features alone for
+        # a model trained without the environment features
+        self.d_in = int(self.x_mean.shape[0])
+        if self.d_in not in (D_IN, D_IN_BASE) or self.params.shape[1] != n_params(self.d_in):
+            raise ValueError("model layer sizes do not match the feature set")
         self.y_mean, self.y_std = np.asarray(y_mean), np.asarray(y_std)  # [type][225]
         self.meta = meta or {}
         self._nets = None
@@ -362,7 +419,7 @@ class LearnedGuess:
 
     def nets(self):
         if self._nets is None:
-            self._nets = [DeviceMlp(p) for p in self.params]
+            self._nets = [DeviceMlp(p, d_in=self.d_in) for p in self.params]
         return self._nets
 
     def close(self):
@@ -372,7 +429,7 @@ class LearnedGuess:
 
     def predict(self, X: np.ndarray, types: np.ndarray) -> np.ndarray:
         """Local-frame blocks (n, 225) for raw features X of pairs of the given types."""
-        Xs = (X - self.x_mean) / self.x_std
+        Xs = (X[:, :self.d_in] - self.x_mean) / self.x_std
         Y = np.zeros((len(X), D_OUT))
         for t, net in enumerate(self.nets()):
             sel = np.nonzero(types == t)[0]
@@ -556,6 +613,8 @@ def _eval_chunk(args):
                      "dE": r_ml.total_energy - r_sad.total_energy,
                      "guess_err_learned": float(np.abs(P0 - r_sad.density).max()),
                      "guess_err_sad": float(np.abs(P_sad - r_sad.density).max()),
+                     "guess_rms_learned": float(np.sqrt(np.mean((P0 - r_sad.density) ** 2))),
+                     "guess_rms_sad": float(np.sqrt(np.mean((P_sad - r_sad.density) ** 2))),
                      "inference_s": t_inf})
     model.close()
     return rows
