@@ -1023,18 +1023,25 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     auto evaluate = [&](int st) {
         const double4* pt = sPt + (st % kVSlots) * kVPts;
         const double4* wv = sWv + (st % kVSlots) * kVPts;
-        for (int idx = tid; idx < mp * kVPts; idx += NT) {
-            const int r = idx / kVPts, p = idx % kVPts;
+        // a thread takes one row at four points (p0, p0 + 4, p0 + 8, p0 + 12):
+        // the row's record, its decoding and its shell centre serve all four
+        for (int idx = tid; idx < mp * 4; idx += NT) {
+            const int r = idx >> 2, p0 = idx & 3;
             // padding rows (shell -1, scale 0) read shell 0 and come out zero
             const RowL row = sRow[r];
             const int sh = max(row.sh, 0);
-            const double4 x = pt[p];
-            const double4 w = wv[p];
-            const AoVal f = ao_from_radial_row(sSh[sh], row, x.x, x.y, x.z, sRad[sh * kVPts + p]);
-            const T v = f.v;
-            const T z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
-            sP[r * LD + p] = v;
-            sZ[r * LD + p] = st * kVPts + p < npts ? z : T(0);
+            const DevShell shl = sSh[sh];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int p = p0 + 4 * k;
+                const double4 x = pt[p];
+                const double4 w = wv[p];
+                const AoVal f = ao_from_radial_row(shl, row, x.x, x.y, x.z, sRad[sh * kVPts + p]);
+                const T v = f.v;
+                const T z = w.x * f.v + w.y * f.gx + w.z * f.gy + w.w * f.gz;
+                sP[r * LD + p] = v;
+                sZ[r * LD + p] = st * kVPts + p < npts ? z : T(0);
+            }
         }
     };
     // MFMA step k takes point 4 kq + k in lane (col, kq): A and B use the
@@ -1193,18 +1200,25 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         const float2* rad = sRad + (st & 1) * nsh * kVPts;
         T* P = sPZ + 2 * (st & 1) * S;
         T* Z = P + S;
-        for (int idx = tid; idx < mp * kVPts; idx += NT) {
-            const int r = idx / kVPts, p = idx % kVPts;
+        // a thread takes one row at four points (p0, p0 + 4, p0 + 8, p0 + 12):
+        // the row's record, its decoding and its shell centre serve all four
+        for (int idx = tid; idx < mp * 4; idx += NT) {
+            const int r = idx >> 2, p0 = idx & 3;
             // padding rows (shell -1, scale 0) read shell 0 and come out zero,
             // and so do points past the group's end (zero (R0, A))
             const RowL row = sRow[r];
             const int sh = max(row.sh, 0);
-            const double4 x = pt[p], wd = wv[p];
-            const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y), static_cast<float>(wd.z),
-                                         static_cast<float>(wd.w));
-            const float2 pz = pz_from_radial_row_f(sSh[sh], row, x.x, x.y, x.z, rad[sh * kVPts + p], w);
-            P[r * LD + p] = pz.x;
-            Z[r * LD + p] = pz.y;
+            const DevShell shl = sSh[sh];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int p = p0 + 4 * k;
+                const double4 x = pt[p], wd = wv[p];
+                const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y),
+                                             static_cast<float>(wd.z), static_cast<float>(wd.w));
+                const float2 pz = pz_from_radial_row_f(shl, row, x.x, x.y, x.z, rad[sh * kVPts + p], w);
+                P[r * LD + p] = pz.x;
+                Z[r * LD + p] = pz.y;
+            }
         }
     };
     using M = Mma<T>;
