@@ -561,18 +561,18 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
         v4f r0 = load(0);
         v4f r1 = load(1);
         __builtin_amdgcn_sched_barrier(0);
-        // leading columns in FP64 with the FP64 density block, four k-steps
-        // per round with their loads issued together
-        const unsigned o64 = static_cast<unsigned>((i0 + f64_row_as_f32(col)) * mp + kq);
+        // leading columns in FP64 with the FP64 density block, in chunks of
+        // 16 whose slots are permuted as in the FP32 part: step s of a chunk
+        // pairs A[i][k + 4 q + s] with Phi[k + 4 q + s], so that lane (i, q)
+        // reads its four density entries as one 32-byte load
+        const unsigned o64 = static_cast<unsigned>((i0 + f64_row_as_f32(col)) * mp + 4 * kq);
         for (int k = 0; k < m64; k += 16) {
-            double av[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) av[s] = P64b[o64 + static_cast<unsigned>(k + 4 * s)];
+            const v4d av = *reinterpret_cast<const v4d*>(P64b + (o64 + static_cast<unsigned>(k)));
+            const double* b = sPhi64 + (k + 4 * kq) * PTS + col;
 #pragma unroll
             for (int s = 0; s < 4; ++s) {
-                const int kr = k + 4 * s + kq;
-                acc64[0] = Mma<double>::op(av[s], sPhi64[kr * PTS + col], acc64[0]);
-                acc64[1] = Mma<double>::op(av[s], sPhi64[(m64 + kr) * PTS + col], acc64[1]);
+                acc64[0] = Mma<double>::op(av[s], b[s * PTS], acc64[0]);
+                acc64[1] = Mma<double>::op(av[s], b[(m64 + s) * PTS], acc64[1]);
             }
         }
         // the FP32 columns in chunks of 16, density loads two chunks ahead
