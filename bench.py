@@ -31,6 +31,7 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # Full-rate VALU issue of the whole chip: 256 CUs x 4 SIMDs, one wave64
 # instruction per SIMD every 4 cycles (16 lanes a cycle) at 2.4 GHz.
 PEAK_VALU_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+PEAK_HBM_GBPS = 8000.0
 E_GATE = 1e-7
 
 
@@ -178,6 +179,11 @@ def load_valu(workload: str):
 # fit it).
 RHO_STAGE_KERNELS = {"fp64": ("k_rho<double, false, 1, 0>",),
                      "fp32": ("k_rho_aoc2", "k_rho<float, true, 1, 2>")}
+# Kernels of the V stage (one launch per group class).
+V_STAGE_KERNELS = {"fp64": ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>"),
+                   "fp32": ("k_vxc_group_pipe<17>", "k_vxc_group_pipe<8>")}
+# Stages whose HBM traffic per build is digested, by the kernel name the roofline reports.
+TRAFFIC_STAGES = {"k_rho": RHO_STAGE_KERNELS, "k_vxc_group": V_STAGE_KERNELS}
 VALU_STAGES = {
     "ao_evaluation_fp64_rho": ("rho", "fp64", ("k_rho<double, false, 1, 0>",)),
     "functional": ("xc", "fp64", ("k_xc<3>",)),
@@ -202,13 +208,14 @@ def _kernel_of(name: str, wanted) -> str:
 
 def _per_stage(rows, counter: str, kernels):
     """(counter total, dispatches, stages) over the dispatches of `kernels`; a stage is one launch per
-    class, the classes told apart by grid size."""
+    class, the classes told apart by kernel and grid size."""
     tot, n, grids = 0.0, 0, set()
     for r in rows:
-        if r["Counter_Name"] == counter and _kernel_of(r["Kernel_Name"], kernels):
+        k = _kernel_of(r["Kernel_Name"], kernels) if r["Counter_Name"] == counter else ""
+        if k:
             tot += float(r["Counter_Value"])
             n += 1
-            grids.add(r["Grid_Size"])
+            grids.add((k, r["Grid_Size"]))
     if grids and n % len(grids):
         raise ValueError(f"{counter}: {n} dispatches of {kernels} do not split into {len(grids)} classes")
     stages = n // max(len(grids), 1)
@@ -216,8 +223,9 @@ def _per_stage(rows, counter: str, kernels):
 
 
 def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str) -> None:
-    """profiles/k_rho_<workload>_<prec>_traffic.json from one FETCH_SIZE and one WRITE_SIZE pass, each
-    scaled by its reading for the 1 GiB device-to-device copy of the same run."""
+    """profiles/<kernel>_<workload>_<prec>_traffic.json (kernel k_rho for the rho stage, k_vxc_group for
+    the V stage) from one FETCH_SIZE and one WRITE_SIZE pass, each scaled by its reading for the 1 GiB
+    device-to-device copy of the same run."""
     fr, wr = _pmc_rows(fetch_dir), _pmc_rows(write_dir)
     copy = 1 << 30
 
@@ -228,21 +236,22 @@ def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str) -> N
 
     fk, wk = copy_kib(fr, "FETCH_SIZE"), copy_kib(wr, "WRITE_SIZE")
     f_scale, w_scale = copy / (fk * 1024.0), copy / (wk * 1024.0)
-    for prec, kernels in RHO_STAGE_KERNELS.items():
+    for kname, prec, kernels in [(k, p, ks) for k, st in TRAFFIC_STAGES.items() for p, ks in st.items()]:
         ft, fn, fs = _per_stage(fr, "FETCH_SIZE", kernels)
         wt, wn, ws = _per_stage(wr, "WRITE_SIZE", kernels)
         if fs == 0 or ws == 0:
             continue
-        out = {"workload": workload, "precision": prec, "kernel": "k_rho", "sources_sha256": sources_digest(),
+        what = "single-spin rho-stage" if kname == "k_rho" else "V-stage"
+        out = {"workload": workload, "precision": prec, "kernel": kname, "sources_sha256": sources_digest(),
                "kernels": list(kernels), "stages": {"fetch": fs, "write": ws},
                "fetch_kib_per_build_raw": ft / fs, "write_kib_per_build_raw": wt / ws,
                "copy_calibration": {"bytes": copy, "fetch_kib": fk, "write_kib": wk, "fetch_scale": f_scale,
                                     "write_scale": w_scale},
                "hbm_bytes_per_build": ft / fs * 1024.0 * f_scale + wt / ws * 1024.0 * w_scale,
                "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one pass each) over {cmd}; sum over the "
-                         "single-spin rho-stage launches of one build (run total / stages), each counter scaled "
+                         f"{what} launches of one build (run total / stages), each counter scaled "
                          "by (1 GiB / its reading for the 1 GiB device-to-device copy kernel)"}
-        path = os.path.join(ROOT, "profiles", f"k_rho_{workload}_{prec}_traffic.json")
+        path = os.path.join(ROOT, "profiles", f"{kname}_{workload}_{prec}_traffic.json")
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
         log(f"wrote {path}: {out['hbm_bytes_per_build'] / 1e6:.1f} MB per build")
@@ -285,6 +294,25 @@ def valu_rooflines(workload: str, kms_by_prec: dict):
                      "achieved": achieved, "peak": PEAK_VALU_INSTR_PER_S, "unit": "wave64 VALU instr/s",
                      "frac": achieved / PEAK_VALU_INSTR_PER_S}
     out["source"] = t["source"]
+    return out
+
+
+def rho_cache_hbm_roofline(workload: str, st0: dict, n_points: int, n_ao: int, rho_ms: float):
+    """HBM roofline of the mixed rho stage read from the AO cache (k_rho_aoc2): its algorithmic bytes are
+    the cache read once (AO values and gradients of every block row and point), (rho, grad rho) written
+    as four doubles per point and P read once; traffic is the measured FETCH + WRITE of the stage."""
+    if not st0.get("ao_cache_bytes"):
+        return None
+    alg = float(st0["ao_cache_bytes"]) + 32.0 * n_points + 8.0 * n_ao * n_ao
+    achieved = alg / (rho_ms * 1e-3) / 1e9
+    t = load_traffic(workload, "fp32", "k_rho")
+    out = {"bound": "hbm", "kernel": "k_rho_aoc2", "launches": "all launches of one build",
+           "algorithmic_bytes_per_build": alg, "stage_ms": rho_ms, "achieved": achieved,
+           "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS,
+           "traffic": t["hbm_bytes_per_build"] if t else None}
+    if t:
+        out["traffic_rate_frac"] = t["hbm_bytes_per_build"] / (rho_ms * 1e-3) / 1e9 / PEAK_HBM_GBPS
+        out["traffic_source"] = t["source"]
     return out
 
 
@@ -641,6 +669,7 @@ def main() -> int:
     log(f"measured {best} MFMA ceiling: {measured_peak:.1f} TFLOP/s (datasheet {peak})")
     traffic = load_traffic(args.workload, best, kname)
     build_tflops = (st["flops_rho"] + st["flops_v"]) / (rb["ms"] * 1e-3) / 1e12
+    rho_hbm = rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"]) if best == "fp32" else None
 
     cpu = None
     parity = None
@@ -708,6 +737,7 @@ def main() -> int:
                      "algorithmic_tflop_per_build": flops[dom] / 1e12, "kernel_ms_per_build": kms[dom],
                      "traffic": traffic["hbm_bytes_per_build"] if traffic else None,
                      "traffic_source": traffic["source"] if traffic else None},
+        "rho_cache_hbm_roofline": rho_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

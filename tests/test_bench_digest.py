@@ -45,6 +45,11 @@ def test_stage_totals_and_copy_calibration(tmp_path, monkeypatch):
         write += [("chx::k_rho_aoc2(chx::RhoArgs)", 512, "WRITE_SIZE", 30.0),
                   ("chx::k_rho_aoc2(chx::RhoArgs)", 256, "WRITE_SIZE", 10.0),
                   ("void chx::k_rho<double, false, 1, 0>(chx::RhoArgs)", 512, "WRITE_SIZE", 5.0)]
+        # V stage: two group kernels launched on the same grid size, one launch each per build
+        fetch += [("void chx::k_vxc_group_pipe<17>(chx::VArgs)", 1024, "FETCH_SIZE", 700.0),
+                  ("void chx::k_vxc_group_pipe<8>(chx::VArgs)", 1024, "FETCH_SIZE", 300.0)]
+        write += [("void chx::k_vxc_group_pipe<17>(chx::VArgs)", 1024, "WRITE_SIZE", 70.0),
+                  ("void chx::k_vxc_group_pipe<8>(chx::VArgs)", 1024, "WRITE_SIZE", 30.0)]
     _write(str(tmp_path / "mf"), fetch)
     _write(str(tmp_path / "mw"), write)
     bench.digest_traffic(str(tmp_path / "mf"), str(tmp_path / "mw"), "water20", "test")
@@ -56,6 +61,10 @@ def test_stage_totals_and_copy_calibration(tmp_path, monkeypatch):
     assert t["sources_sha256"] == digest
     f64 = json.load(open(tmp_path / "profiles" / "k_rho_water20_fp64_traffic.json"))
     assert f64["hbm_bytes_per_build"] == pytest.approx((50.0 * 2 + 5.0) * 1024)
+    v = json.load(open(tmp_path / "profiles" / "k_vxc_group_water20_fp32_traffic.json"))
+    assert v["kernel"] == "k_vxc_group" and v["stages"] == {"fetch": 2, "write": 2}
+    assert v["hbm_bytes_per_build"] == pytest.approx((1000.0 * 2 + 100.0) * 1024)
+    assert not os.path.exists(tmp_path / "profiles" / "k_vxc_group_water20_fp64_traffic.json")
 
 
 def test_uneven_class_dispatches_are_refused(tmp_path):
