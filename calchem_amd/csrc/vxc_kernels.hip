@@ -470,10 +470,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
 // points), so the density blocks are read half as often as by one sub-tile
 // at a time.  LDS holds the B operands of both sub-tiles: Phi of the FP64
 // leading rows (below m64) and of the FP32 rows (from m64 on).  Wave w takes
-// the 16-row tiles w, w + 4, ...; the epilogue reads each tile's (phi, grad
-// phi) of both sub-tiles from the cache a second time (from L2): for an FP32
-// row tile before its contraction, so that the loads arrive during it, for
-// a leading row tile (32-byte slots) after it.  The sums are those of
+// the 16-row tiles w, w + 4, ...; after each tile's contraction the epilogue
+// reads the tile's gradient slots of one sub-tile, then the other, from the
+// cache's gradient plane and phi from LDS, so the cache is read once.
+// (Requesting the gradient slots before the contraction was slower: 0.570
+// against 0.549 ms per 20-water stage; overlaying the sums on the operands,
+// for less LDS per workgroup, made no difference.  In the two-spin
+// k_rho<float, true, 2, kAocLoad>, phi from LDS made the unrestricted build
+// slower, 3.22 against 3.15 ms, so that kernel reads it from the cache.)  The sums are those of
 // k_rho<float, true, 1, kAocLoad>, formed in the same order.
 __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -625,7 +629,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const double* g = reinterpret_cast<const double*>(grad_at(it, v, s));
-                    w[v] = AoVal{*reinterpret_cast<const double*>(phi_at(it, v, s)), g[0], g[1], g[2]};
+                    w[v] = AoVal{sPhi64[(s * m64 + (it << 4) + 4 * kq + v) * PTS + col], g[0], g[1], g[2]};
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v) accumulate(s, acc[s], acc64[s], v, w[v]);
@@ -639,7 +643,7 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const float* g = reinterpret_cast<const float*>(grad_at(it, v, s));
-                    w[v] = make_float4(*reinterpret_cast<const float*>(phi_at(it, v, s)), g[0], g[1], g[2]);
+                    w[v] = make_float4(sPhi[(s * mr + (it << 4) - m64 + 4 * kq + v) * PTS + col], g[0], g[1], g[2]);
                 }
 #pragma unroll
                 for (int v = 0; v < 4; ++v)
