@@ -54,6 +54,45 @@ struct Mma<float> {
     }
 };
 
+// BF16 operands of v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][8 (l / 16) + e]
+// and B[8 (l / 16) + e][l % 16], e = 0..7, element e in half e % 2 of dword
+// e / 2; the result D[4 (l / 16) + v][l % 16] lands as in the FP32 16x16x4 form.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned v4u __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4f mfma_bf16(const v4u& a, const v4u& b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a), __builtin_bit_cast(bf16x8, b), c,
+                                                   0, 0, 0);
+}
+
+// (x, y) rounded to BF16 (nearest, ties to even) in one dword, x in the low half.
+__device__ __forceinline__ unsigned pack_bf16(float x, float y) {
+    const bf16x2 h = {static_cast<__bf16>(x), static_cast<__bf16>(y)};
+    return __builtin_bit_cast(unsigned, h);
+}
+
+// Three-term BF16 split of the pair (x, y): x = x0 + x1 + x2 + e with each
+// term the nearest BF16 to what the terms before it leave (every remainder is
+// exact in FP32), |e| <= 2^-24 |x| up to FP32 underflow; the same for y.  The
+// products of two such splits that reach FP32 resolution are the six with
+// term indices summing to at most 2.
+struct Bf16Split3 {
+    unsigned t[3];
+};
+__device__ __forceinline__ Bf16Split3 split3_bf16(float x, float y) {
+    Bf16Split3 o;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        o.t[k] = pack_bf16(x, y);
+        if (k < 2) {
+            x -= __builtin_bit_cast(float, o.t[k] << 16);
+            y -= __builtin_bit_cast(float, o.t[k] & 0xffff0000u);
+        }
+    }
+    return o;
+}
+
 template <typename T>
 struct Vec4;
 template <>
@@ -1102,19 +1141,38 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
     }
 }
 
-// Pipelined form of k_vxc_group for the mixed build (FP32 tiles, 16 waves):
-// one barrier per step of 16 points.  Between two barriers the workgroup
-// does four independent things: the MFMA updates of step s (phi / Z buffer
-// s % 2), the AO values of step s + 1 into the other buffer, the radial sums
-// of step s + 2 (radial buffer s % 2) and the loads of the points and wv of
-// step s + 3 (slot (s + 3) % 4 of four).  Each buffer is written in one
-// interval and read in a later one, so the barrier between them is the only
-// ordering needed; waves that finish their MFMA share early go on with AO
-// evaluation, so MFMA and VALU work overlap on the SIMDs.
+// Pipelined form of k_vxc_group for the mixed build (FP32 AO values, 16
+// waves): one barrier per step of 16 points.  Between two barriers the
+// workgroup does four independent things: the MFMA updates of step s
+// (operand buffer s % 2), the AO values of step s + 1 into the other buffer,
+// the radial sums of step s + 2 (radial buffer s % 2) and the loads of the
+// points and wv of step s + 3 (slot (s + 3) % 4 of four).  Each buffer is
+// written in one interval and read in a later one, so the barrier between
+// them is the only ordering needed; waves that finish their MFMA share early
+// go on with AO evaluation, so MFMA and VALU work overlap on the SIMDs.
+//
+// The tile-pair update Z_I P_J^T + P_I Z_J^T of a step is one 16 x 16 x 32
+// product over k = 2 p + s (point p, s = 0, 1): A[i][2p] = Z_I[i][p],
+// A[i][2p + 1] = P_I[i][p], B[2p][j] = P_J[j][p], B[2p + 1][j] = Z_J[j][p].
+// P and Z are formed in FP32 and split in three BF16 terms each
+// (split3_bf16), and the six products of terms whose indices sum to at most
+// 2 run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation: FP32 operand
+// resolution at a sixth of the matrix-core cycles of the FP32 16x16x4 form
+// (6 x 16 against 8 x 32 cycles per pair and step).  LDS holds per buffer and
+// term one dword (P, Z) per (row, point): 16 dwords per row, the 16-byte chunk
+// of points 4c .. 4c + 3 stored at chunk c ^ ((row / 4) % 4), so that the 16
+// rows of an operand read fall on distinct bank groups.  A reads the same
+// dwords with the halves swapped.
+// Point slots of the pipelined kernel: in the interval of step s it reads the
+// points of steps s + 1 and s + 2 and loads those of step s + 3, so three
+// slots suffice; wv is kept as the FP32 values the kernel uses.
+constexpr int kVPipeSlots = 3;
+
 template <int MAXT>
 __host__ __device__ constexpr size_t vgroup_pipe_lds_fixed() {
-    return sizeof(double4) * 2 * kVSlots * kVPts + sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
-           sizeof(RowL) * MAXT * 16 + sizeof(float) * 4 * MAXT * 16 * vgroup_ld<float>();
+    return (sizeof(double4) + sizeof(float4)) * kVPipeSlots * kVPts +
+           sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
+           sizeof(RowL) * MAXT * 16 + sizeof(unsigned) * 2 * 3 * MAXT * 16 * kVPts;
 }
 
 // Waves of the pipelined kernel: 16 for groups of up to 12 or 17 tiles, 8
@@ -1130,23 +1188,23 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
     extern __shared__ __align__(16) unsigned char smem[];
     using T = float;
     constexpr int NW = vgroup_pipe_waves<MAXT>(), NT = NW * 64, PPW = (vgroup_max_pairs<MAXT>() + NW - 1) / NW;
-    constexpr int LD = vgroup_ld<T>();
+    static_assert(kVPts == 16, "one 16-dword row of (P, Z) per 16 points");
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh, npts = G.n_pts;
     const int T16 = (m + 15) >> 4, mp = T16 << 4;
     const int npairs = T16 * (T16 + 1) / 2;
     const int nsteps = (npts + kVPts - 1) / kVPts;
     unsigned char* q = smem;
-    double4* sPt = reinterpret_cast<double4*>(q);  // [kVSlots][kVPts] points, then [kVSlots][kVPts] wv
-    double4* sWv = sPt + kVSlots * kVPts;
-    q += sizeof(double4) * 2 * kVSlots * kVPts;
+    double4* sPt = reinterpret_cast<double4*>(q);                 // [kVPipeSlots][kVPts] points
+    float4* sWv = reinterpret_cast<float4*>(sPt + kVPipeSlots * kVPts);  // [kVPipeSlots][kVPts] wv
+    q += (sizeof(double4) + sizeof(float4)) * kVPipeSlots * kVPts;
     int2* sPair = reinterpret_cast<int2*>(q);
     q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
     q += sizeof(RowL) * MAXT * 16;
-    T* sPZ = reinterpret_cast<T*>(q);  // buffer b: phi at sPZ + 2 b S, Z at sPZ + (2 b + 1) S
-    constexpr int S = MAXT * 16 * LD;
-    q += sizeof(T) * 4 * S;
+    unsigned* sW = reinterpret_cast<unsigned*>(q);  // buffer b, term t at sW + (3 b + t) S: [row][16 points]
+    constexpr int S = MAXT * 16 * kVPts;
+    q += sizeof(unsigned) * 6 * S;
     float2* sRad = reinterpret_cast<float2*>(q);  // buffer b at sRad + b nsh kVPts
     q += sizeof(float2) * 2 * nsh * kVPts;
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
@@ -1168,73 +1226,80 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         int ti = 0;
         while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
         const int tj = p - ti * (ti + 1) / 2;
-        sPair[p] = make_int2(ti * 16 * LD, tj * 16 * LD);
+        sPair[p] = make_int2(ti * 16 * kVPts, tj * 16 * kVPts);
     }
 
     // points and wv of step st into its slot (zero past the group's points)
     auto load_points = [&](int st) {
         if (tid < 2 * kVPts) {
-            const int j = tid % kVPts, g = st * kVPts + j, slot = st % kVSlots;
+            const int j = tid % kVPts, g = st * kVPts + j, slot = st % kVPipeSlots;
             double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
             if (g < npts) {
                 const int sg = G.pt_begin + g;
                 v = tid < kVPts ? a.pts[sg] : a.wv[a.wv_perm ? a.perm[sg] : sg];
             }
-            (tid < kVPts ? sPt : sWv)[slot * kVPts + j] = v;
+            if (tid < kVPts)
+                sPt[slot * kVPts + j] = v;
+            else
+                sWv[slot * kVPts + j] = make_float4(static_cast<float>(v.x), static_cast<float>(v.y),
+                                                    static_cast<float>(v.z), static_cast<float>(v.w));
         }
     };
     // (R0, A) per (shell, point) of step st, A = w0 R0 + R1 (w . d): the
     // points and wv of step st arrived two intervals earlier
     auto radial = [&](int st) {
-        const double4* pt = sPt + (st % kVSlots) * kVPts;
-        const double4* wv = sWv + (st % kVSlots) * kVPts;
+        const double4* pt = sPt + (st % kVPipeSlots) * kVPts;
+        const float4* wv = sWv + (st % kVPipeSlots) * kVPts;
         float2* rad = sRad + (st & 1) * nsh * kVPts;
         for (int idx = tid; idx < nsh * kVPts; idx += NT) {
             const int sh = idx / kVPts, p = idx % kVPts;
-            const double4 x = pt[p], wd = wv[p];
-            const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y), static_cast<float>(wd.z),
-                                         static_cast<float>(wd.w));
+            const double4 x = pt[p];
+            const float4 w = wv[p];
             rad[idx] = st * kVPts + p < npts ? shell_radial_za_f(sSh[sh], sPrim, x.x, x.y, x.z, w)
                                              : make_float2(0.0f, 0.0f);
         }
     };
     auto evaluate = [&](int st) {
-        const double4* pt = sPt + (st % kVSlots) * kVPts;
-        const double4* wv = sWv + (st % kVSlots) * kVPts;
+        const double4* pt = sPt + (st % kVPipeSlots) * kVPts;
+        const float4* wv = sWv + (st % kVPipeSlots) * kVPts;
         const float2* rad = sRad + (st & 1) * nsh * kVPts;
-        T* P = sPZ + 2 * (st & 1) * S;
-        T* Z = P + S;
-        // a thread takes one row at four points (p0, p0 + 4, p0 + 8, p0 + 12):
-        // the row's record, its decoding and its shell centre serve all four
+        unsigned* W = sW + 3 * (st & 1) * S;
+        // a thread takes one row at the four points 4 c .. 4 c + 3: the row's
+        // record, its decoding and its shell centre serve all four, and each
+        // term's four dwords go to LDS as one 16-byte store
         for (int idx = tid; idx < mp * 4; idx += NT) {
-            const int r = idx >> 2, p0 = idx & 3;
+            const int r = idx >> 2, c = idx & 3;
             // padding rows (shell -1, scale 0) read shell 0 and come out zero,
             // and so do points past the group's end (zero (R0, A))
             const RowL row = sRow[r];
             const int sh = max(row.sh, 0);
             const DevShell shl = sSh[sh];
+            v4u t[3];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int p = p0 + 4 * k;
-                const double4 x = pt[p], wd = wv[p];
-                const float4 w = make_float4(static_cast<float>(wd.x), static_cast<float>(wd.y),
-                                             static_cast<float>(wd.z), static_cast<float>(wd.w));
+                const int p = 4 * c + k;
+                const double4 x = pt[p];
+                const float4 w = wv[p];
                 const float2 pz = pz_from_radial_row_f(shl, row, x.x, x.y, x.z, rad[sh * kVPts + p], w);
-                P[r * LD + p] = pz.x;
-                Z[r * LD + p] = pz.y;
+                const Bf16Split3 sp = split3_bf16(pz.x, pz.y);
+#pragma unroll
+                for (int j = 0; j < 3; ++j) t[j][k] = sp.t[j];
             }
+            const int at = r * kVPts + 4 * (c ^ ((r >> 2) & 3));
+#pragma unroll
+            for (int j = 0; j < 3; ++j) *reinterpret_cast<v4u*>(W + j * S + at) = t[j];
         }
     };
     using M = Mma<T>;
-    using V4 = typename Vec4<T>::type;
     typename M::acc acc[PPW];
 #pragma unroll
     for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
+    // this lane's 16-byte chunk of a tile row: points 4 kq .. 4 kq + 3 of row col
+    const int lane_at = col * kVPts + 4 * (kq ^ ((col >> 2) & 3));
     auto update = [&](int st) {
-        const T* P = sPZ + 2 * (st & 1) * S;
-        const T* Z = P + S;
+        const unsigned* W = sW + 3 * (st & 1) * S;
         int cur_i = -1;
-        V4 zi, pi;
+        v4u a[3];
 #pragma unroll
         for (int w = 0; w < PPW; ++w) {
             const int p = wave * per_wave + w;
@@ -1242,23 +1307,27 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
             const int2 pr = sPair[p];
             if (w == 0 || pr.x != cur_i) {
                 cur_i = pr.x;
-                const int ri = pr.x + col * LD + 4 * kq;
-                zi = lds_load4(Z + ri);
-                pi = lds_load4(P + ri);
-            }
-            const int rj = pr.y + col * LD + 4 * kq;
-            const V4 pj = lds_load4(P + rj);
-            const V4 zj = lds_load4(Z + rj);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                acc[w] = M::op(zi[k], pj[k], acc[w]);
-                acc[w] = M::op(pi[k], zj[k], acc[w]);
+                for (int j = 0; j < 3; ++j) {
+                    const v4u v = *reinterpret_cast<const v4u*>(W + j * S + pr.x + lane_at);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) a[j][e] = __builtin_amdgcn_alignbit(v[e], v[e], 16);  // (Z, P)
+                }
             }
+            v4u b[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) b[j] = *reinterpret_cast<const v4u*>(W + j * S + pr.y + lane_at);
+            acc[w] = mfma_bf16(a[1], b[1], acc[w]);
+            acc[w] = mfma_bf16(a[2], b[0], acc[w]);
+            acc[w] = mfma_bf16(a[0], b[2], acc[w]);
+            acc[w] = mfma_bf16(a[1], b[0], acc[w]);
+            acc[w] = mfma_bf16(a[0], b[1], acc[w]);
+            acc[w] = mfma_bf16(a[0], b[0], acc[w]);
         }
     };
 
     // prologue: points of steps 0-2, radial sums of steps 0-1, AO values of step 0
-    for (int st = 0; st < 3 && st < nsteps; ++st) load_points(st);
+    for (int st = 0; st < kVPipeSlots && st < nsteps; ++st) load_points(st);
     __syncthreads();
     radial(0);
     if (nsteps > 1) radial(1);
