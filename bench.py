@@ -32,6 +32,8 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # instruction per SIMD every 4 cycles (16 lanes a cycle) at 2.4 GHz.
 PEAK_VALU_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 PEAK_HBM_GBPS = 8000.0
+# Shapes the bench tries per precision: (block_points, group_points, group_max_rows), 0 = library default.
+TUNE_CANDIDATES = ((0, 0, 0), (256, 384, 0), (384, 384, 0), (320, 512, 0), (512, 512, 0))
 E_GATE = 1e-7
 
 
@@ -297,7 +299,8 @@ def valu_rooflines(workload: str, kms_by_prec: dict):
     return out
 
 
-def rho_cache_hbm_roofline(workload: str, st0: dict, n_points: int, n_ao: int, rho_ms: float):
+def rho_cache_hbm_roofline(workload: str, st0: dict, n_points: int, n_ao: int, rho_ms: float,
+                           default_shapes: bool = True):
     """HBM roofline of the mixed rho stage read from the AO cache (k_rho_aoc2): its algorithmic bytes are
     the cache read once (AO values and gradients of every block row and point), (rho, grad rho) written
     as four doubles per point and P read once; traffic is the measured FETCH + WRITE of the stage."""
@@ -305,7 +308,7 @@ def rho_cache_hbm_roofline(workload: str, st0: dict, n_points: int, n_ao: int, r
         return None
     alg = float(st0["ao_cache_bytes"]) + 32.0 * n_points + 8.0 * n_ao * n_ao
     achieved = alg / (rho_ms * 1e-3) / 1e9
-    t = load_traffic(workload, "fp32", "k_rho")
+    t = load_traffic(workload, "fp32", "k_rho") if default_shapes else None
     out = {"bound": "hbm", "kernel": "k_rho_aoc2", "launches": "all launches of one build",
            "algorithmic_bytes_per_build": alg, "stage_ms": rho_ms, "achieved": achieved,
            "peak": PEAK_HBM_GBPS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS,
@@ -498,6 +501,8 @@ def main() -> int:
                     help="max |phi| above which the mixed build keeps an AO row FP64 (0 = library default)")
     ap.add_argument("--ao-cache", type=int, default=0, choices=[0, 1, 2],
                     help="mixed build's AO cache: 0 auto, 1 off, 2 on")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use the library's default shapes instead of the autotuner's pick per precision")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one device-to-device copy of this size (MiB) so HBM counters can be scaled")
     ap.add_argument("--digest-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"), default=None,
@@ -538,16 +543,41 @@ def main() -> int:
     P = configs.random_density(basis, w["n_occ"], w["seed"], S)
     log(f"{args.workload}: {mol.n_atoms} atoms, {basis.n_ao} AO, {grid.n_points} points "
         f"(setup {time.perf_counter() - t0:.1f}s)")
-    t0 = time.perf_counter()
-    eng = RksVxcEngine(basis, grid, args.block_points, args.group_points,
-                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows,
-                       **plan_extra(args))
-    t_plan = time.perf_counter() - t0
-    st0 = eng.stats()
-    log(f"plan {t_plan:.2f}s: {st0['n_blocks']} blocks, {st0['n_groups']} groups, rho tiles "
-        f"{st0['n_rho_tiles']}, V pairs {st0['n_v_pairs']}, max m {st0['max_m_rho']}/{st0['max_m_v']}, "
-        f"flops rho {st0['flops_rho']:.3e} v {st0['flops_v']:.3e}, device {st0['device_bytes'] / 2**30:.2f} GiB "
-        f"(AO cache {st0['ao_cache_bytes'] / 2**30:.2f} GiB)")
+    # Block / group shapes per precision: the autotuner's pick among a few
+    # candidates (the library default kept unless another is > 2 % faster),
+    # or the caller's shapes.
+    given = (args.block_points, args.group_points, args.group_max_rows)
+    shapes = {"fp64": given, "fp32": given}
+    tune = None
+    if (not args.no_autotune and given == (0, 0, 0) and args.rho_tile_points == 0 and args.ao_cache == 0
+            and args.mixed_threshold == 0.0 and args.copy_calibration_mib == 0):
+        from calchem_amd.autotune import autotune
+        tune = {}
+        for prec in ("fp64", "fp32"):
+            t0 = time.perf_counter()
+            r = autotune(basis, grid, func, P, prec, TUNE_CANDIDATES, steps=5, warmup=1, keep_first_within=0.02)
+            shapes[prec] = (r.block_points, r.group_points, r.group_max_rows)
+            tune[prec] = {"picked": list(shapes[prec]), "seconds": time.perf_counter() - t0,
+                          "table": [[list(c), ms, e] for c, ms, e in r.table]}
+            log(f"autotune {prec}: " + ", ".join(f"{c} {ms:.3f} ms" for c, ms, _ in r.table) +
+                f" -> {shapes[prec]} ((0, 0, 0) = library default)")
+    engs = {}
+    for prec in ("fp64", "fp32"):
+        if shapes[prec] in [shapes[q] for q in engs]:
+            engs[prec] = engs[next(q for q in engs if shapes[q] == shapes[prec])]
+            continue
+        bp, gp, gr = shapes[prec]
+        t0 = time.perf_counter()
+        engs[prec] = RksVxcEngine(basis, grid, bp, gp, rho_tile_points=args.rho_tile_points, group_max_rows=gr,
+                                  **plan_extra(args))
+        t_plan = time.perf_counter() - t0
+        s0 = engs[prec].stats()
+        log(f"plan {shapes[prec]} {t_plan:.2f}s: {s0['n_blocks']} blocks, {s0['n_groups']} groups, rho tiles "
+            f"{s0['n_rho_tiles']}, V pairs {s0['n_v_pairs']}, max m {s0['max_m_rho']}/{s0['max_m_v']}, "
+            f"flops rho {s0['flops_rho']:.3e} v {s0['flops_v']:.3e}, device {s0['device_bytes'] / 2**30:.2f} GiB "
+            f"(AO cache {s0['ao_cache_bytes'] / 2**30:.2f} GiB)")
+    eng = engs["fp64"]
+    st0 = engs["fp32"].stats()
 
     n = basis.n_ao
     stream = hr.Stream()
@@ -565,6 +595,7 @@ def main() -> int:
 
     results = {}
     for prec in ("fp64", "fp32"):
+        eng = engs[prec]
         eng.build_device(func, dP.address, dV.address, den.address, stream.address, prec)
         stream.synchronize()
         V = dV.to_numpy((n, n))
@@ -593,6 +624,7 @@ def main() -> int:
             f"E_xc={en[0]:.12f} N={en[1]:.10f}, kernels {kms}")
 
     # XC nuclear gradient of the same workload (restricted, FP64), host buffers.
+    eng = engs["fp64"]
     eng.set_atoms(mol, grid)
     eng.xc_gradient(func, P)
     t0 = time.perf_counter()
@@ -637,11 +669,11 @@ def main() -> int:
     # Host-buffer path (P copied in, V copied out over PCIe, synchronous), for reference only.
     pcie = {}
     for prec in ("fp64", "fp32"):
-        eng.evaluate_full(func, P, prec)
+        engs[prec].evaluate_full(func, P, prec)
         t0 = time.perf_counter()
         reps = max(3, min(args.steps, 10))
         for _ in range(reps):
-            eng.evaluate_full(func, P, prec)
+            engs[prec].evaluate_full(func, P, prec)
         pcie[prec] = reps / (time.perf_counter() - t0)
     log(f"host-buffer (PCIe-inclusive) builds/s: {pcie}")
 
@@ -667,9 +699,11 @@ def main() -> int:
     peak = PEAK_TFLOPS[best]
     measured_peak = measured_mfma_peak(best)
     log(f"measured {best} MFMA ceiling: {measured_peak:.1f} TFLOP/s (datasheet {peak})")
-    traffic = load_traffic(args.workload, best, kname)
+    # the committed traffic digests are measured on the library's default shapes
+    traffic = load_traffic(args.workload, best, kname) if shapes[best] == (0, 0, 0) else None
     build_tflops = (st["flops_rho"] + st["flops_v"]) / (rb["ms"] * 1e-3) / 1e12
-    rho_hbm = rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"]) if best == "fp32" else None
+    rho_hbm = (rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"], shapes["fp32"] == (0, 0, 0))
+               if best == "fp32" else None)
 
     cpu = None
     parity = None
@@ -715,7 +749,10 @@ def main() -> int:
                                f"{w['n_rad']}x{w['n_ang']} Becke grid ({grid.n_points} points), "
                                f"{func.upper()}, random S-orthonormal P ({w['n_occ']} occ)",
                    "precision": best, "func": func, "n_ao": n, "n_points": grid.n_points,
-                   "ao_cache_bytes": st0["ao_cache_bytes"]},
+                   "ao_cache_bytes": st0["ao_cache_bytes"],
+                   "shapes": {q: list(shapes[q]) for q in shapes},
+                   "shapes_note": "(block_points, group_points, group_max_rows); 0 = library default (320, 384, 272)"},
+        "autotune": tune,
         "fp64_builds_per_s": 1000.0 / r64["ms"],
         "fp32_builds_per_s": 1000.0 / r32["ms"],
         "mixed_vs_fp64_speedup": r64["ms"] / r32["ms"],

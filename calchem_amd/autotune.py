@@ -21,10 +21,10 @@ from .basis import BasisSet
 from .grid import MolecularGrid
 from .vxc import RksVxcEngine
 
-# (block_points, group_points, group_max_rows); 0 = library default.
+# (block_points, group_points, group_max_rows); 0 = library default (320 / 384 / 272).
 DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
-    (256, 384, 0), (256, 256, 0), (256, 512, 0), (256, 1024, 0), (192, 384, 0), (128, 384, 0),
-    (256, 1024, 192),
+    (0, 0, 0), (256, 384, 0), (384, 384, 0), (512, 512, 0), (320, 512, 0), (320, 256, 0), (192, 384, 0),
+    (320, 1024, 192),
 )
 
 
@@ -43,9 +43,12 @@ class TuneResult:
 
 def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, precision: str = "fp64",
              candidates: Optional[Iterable[Tuple[int, int, int]]] = None, steps: int = 5, warmup: int = 1,
-             e_tol: Optional[float] = None) -> TuneResult:
+             e_tol: Optional[float] = None, keep_first_within: float = 0.0) -> TuneResult:
     """e_tol: largest accepted |E_xc - E_xc(first candidate)|; default 1e-9 Eh for
-    FP64 builds, 1e-7 Eh for mixed ones (their rounding depends on the blocking)."""
+    FP64 builds, 1e-7 Eh for mixed ones (their rounding depends on the blocking).
+    keep_first_within: the first candidate is kept unless another is faster by
+    more than this fraction of its time (so that run-to-run spread between
+    near-equal shapes does not change the plan)."""
     if e_tol is None:
         e_tol = 1e-9 if precision == "fp64" else 1e-7
     cands = list(candidates or DEFAULT_CANDIDATES)
@@ -77,4 +80,6 @@ def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, pre
         table.append(((bp, gp, gr), ms, e))
     ok = [t for t in table if abs(t[2] - e_first) <= e_tol]
     (bp, gp, gr), ms, _ = min(ok, key=lambda t: t[1])
+    if ms >= table[0][1] * (1.0 - keep_first_within):
+        (bp, gp, gr), ms, _ = table[0]
     return TuneResult(bp, gp, gr, ms, table)

@@ -305,7 +305,7 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     P.n_in = n_pts;
     P.has_weights = has_w;
     P.n_mol = n_mol;
-    const int nb_rho = (opt && opt->block_points > 0) ? round_up(opt->block_points, 32) : 256;
+    const int nb_rho = (opt && opt->block_points > 0) ? round_up(opt->block_points, 32) : kDefaultBlockPoints;
     const int nb_v = (opt && opt->group_points > 0) ? opt->group_points : 384;
     const size_t mem_limit = opt ? opt->memory_limit_bytes : 0;
     if (opt && opt->mixed_exact_threshold > 0.0) P.mixed_threshold = opt->mixed_exact_threshold;

@@ -74,7 +74,7 @@ enum {
 };
 
 typedef struct {
-    int32_t block_points;          /* max points per rho-stage block (default 256) */
+    int32_t block_points;          /* max points per rho-stage block (default 320) */
     int32_t group_points;          /* target points per V-stage group (default 384) */
     size_t memory_limit_bytes;     /* 0 = no limit */
     double mixed_exact_threshold;  /* mixed builds keep AOs with max |phi| above this in FP64 (0 = 0.1) */
