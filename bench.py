@@ -32,8 +32,9 @@ PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 # instruction per SIMD every 4 cycles (16 lanes a cycle) at 2.4 GHz.
 PEAK_VALU_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
 PEAK_HBM_GBPS = 8000.0
-# Shapes the bench tries per precision: (block_points, group_points, group_max_rows), 0 = library default.
-TUNE_CANDIDATES = ((0, 0, 0), (256, 384, 0), (384, 384, 0), (320, 512, 0), (512, 512, 0))
+# Plans the bench tries per precision: (block_points, group_points, group_max_rows, v_small_bf16),
+# 0 = library default; the operand form (last entry) only matters for mixed builds.
+TUNE_CANDIDATES = ((0, 0, 0, 0), (0, 0, 0, 2), (256, 384, 0, 0), (256, 384, 0, 2), (0, 512, 0, 0), (0, 512, 0, 2))
 E_GATE = 1e-7
 
 
@@ -501,6 +502,8 @@ def main() -> int:
                     help="max |phi| above which the mixed build keeps an AO row FP64 (0 = library default)")
     ap.add_argument("--ao-cache", type=int, default=0, choices=[0, 1, 2],
                     help="mixed build's AO cache: 0 auto, 1 off, 2 on")
+    ap.add_argument("--v-small-bf16", type=int, default=0, choices=[0, 1, 2],
+                    help="mixed small-group V operands: 0/1 FP32, 2 BF16 three-term")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use the library's default shapes instead of the autotuner's pick per precision")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
@@ -546,30 +549,31 @@ def main() -> int:
     # Block / group shapes per precision: the autotuner's pick among a few
     # candidates (the library default kept unless another is > 2 % faster),
     # or the caller's shapes.
-    given = (args.block_points, args.group_points, args.group_max_rows)
+    given = (args.block_points, args.group_points, args.group_max_rows, args.v_small_bf16)
     shapes = {"fp64": given, "fp32": given}
     tune = None
-    if (not args.no_autotune and given == (0, 0, 0) and args.rho_tile_points == 0 and args.ao_cache == 0
+    if (not args.no_autotune and given == (0, 0, 0, 0) and args.rho_tile_points == 0 and args.ao_cache == 0
             and args.mixed_threshold == 0.0 and args.copy_calibration_mib == 0):
         from calchem_amd.autotune import autotune
         tune = {}
         for prec in ("fp64", "fp32"):
             t0 = time.perf_counter()
-            r = autotune(basis, grid, func, P, prec, TUNE_CANDIDATES, steps=5, warmup=1, keep_first_within=0.02)
-            shapes[prec] = (r.block_points, r.group_points, r.group_max_rows)
+            cands = [c for c in TUNE_CANDIDATES if prec == "fp32" or c[3] == 0]
+            r = autotune(basis, grid, func, P, prec, cands, steps=5, warmup=1, keep_first_within=0.02)
+            shapes[prec] = r.shape
             tune[prec] = {"picked": list(shapes[prec]), "seconds": time.perf_counter() - t0,
                           "table": [[list(c), ms, e] for c, ms, e in r.table]}
             log(f"autotune {prec}: " + ", ".join(f"{c} {ms:.3f} ms" for c, ms, _ in r.table) +
-                f" -> {shapes[prec]} ((0, 0, 0) = library default)")
+                f" -> {shapes[prec]} ((0, 0, 0, 0) = library default)")
     engs = {}
     for prec in ("fp64", "fp32"):
         if shapes[prec] in [shapes[q] for q in engs]:
             engs[prec] = engs[next(q for q in engs if shapes[q] == shapes[prec])]
             continue
-        bp, gp, gr = shapes[prec]
+        bp, gp, gr, vs = shapes[prec]
         t0 = time.perf_counter()
         engs[prec] = RksVxcEngine(basis, grid, bp, gp, rho_tile_points=args.rho_tile_points, group_max_rows=gr,
-                                  **plan_extra(args))
+                                  v_small_bf16=vs, **plan_extra(args))
         t_plan = time.perf_counter() - t0
         s0 = engs[prec].stats()
         log(f"plan {shapes[prec]} {t_plan:.2f}s: {s0['n_blocks']} blocks, {s0['n_groups']} groups, rho tiles "
@@ -699,10 +703,12 @@ def main() -> int:
     peak = PEAK_TFLOPS[best]
     measured_peak = measured_mfma_peak(best)
     log(f"measured {best} MFMA ceiling: {measured_peak:.1f} TFLOP/s (datasheet {peak})")
-    # the committed traffic digests are measured on the library's default shapes
-    traffic = load_traffic(args.workload, best, kname) if shapes[best] == (0, 0, 0) else None
+    # the committed traffic digests are measured on the library's default plan; the rho stage's
+    # traffic depends only on the blocks (block_points), the V stage's on the whole plan
+    dflt = shapes[best][0] == 0 if kname == "k_rho" else shapes[best] == (0, 0, 0, 0)
+    traffic = load_traffic(args.workload, best, kname) if dflt else None
     build_tflops = (st["flops_rho"] + st["flops_v"]) / (rb["ms"] * 1e-3) / 1e12
-    rho_hbm = (rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"], shapes["fp32"] == (0, 0, 0))
+    rho_hbm = (rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"], shapes["fp32"][0] == 0)
                if best == "fp32" else None)
 
     cpu = None
@@ -751,7 +757,8 @@ def main() -> int:
                    "precision": best, "func": func, "n_ao": n, "n_points": grid.n_points,
                    "ao_cache_bytes": st0["ao_cache_bytes"],
                    "shapes": {q: list(shapes[q]) for q in shapes},
-                   "shapes_note": "(block_points, group_points, group_max_rows); 0 = library default (320, 384, 272)"},
+                   "shapes_note": "(block_points, group_points, group_max_rows, v_small_bf16); 0 = library default "
+                                  "(320, 384, 272, FP32 small-group V operands)"},
         "autotune": tune,
         "fp64_builds_per_s": 1000.0 / r64["ms"],
         "fp32_builds_per_s": 1000.0 / r32["ms"],

@@ -30,7 +30,7 @@ class VxcOptions(C.Structure):
     _fields_ = [("block_points", C.c_int32), ("group_points", C.c_int32),
                 ("memory_limit_bytes", C.c_size_t), ("mixed_exact_threshold", C.c_double),
                 ("rho_tile_points", C.c_int32), ("group_max_rows", C.c_int32), ("ao_cache", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("v_small_bf16", C.c_int32)]
 
 
 class VxcStats(C.Structure):

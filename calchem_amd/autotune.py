@@ -21,10 +21,11 @@ from .basis import BasisSet
 from .grid import MolecularGrid
 from .vxc import RksVxcEngine
 
-# (block_points, group_points, group_max_rows); 0 = library default (320 / 384 / 272).
-DEFAULT_CANDIDATES: Tuple[Tuple[int, int, int], ...] = (
-    (0, 0, 0), (256, 384, 0), (384, 384, 0), (512, 512, 0), (320, 512, 0), (320, 256, 0), (192, 384, 0),
-    (320, 1024, 192),
+# (block_points, group_points, group_max_rows[, v_small_bf16]); 0 = library
+# default (320 / 384 / 272 / FP32 small-group V operands).
+DEFAULT_CANDIDATES: Tuple[Tuple[int, ...], ...] = (
+    (0, 0, 0), (0, 0, 0, 2), (256, 384, 0), (384, 384, 0), (512, 512, 0), (320, 512, 0), (320, 256, 0),
+    (192, 384, 0), (320, 1024, 192),
 )
 
 
@@ -34,15 +35,20 @@ class TuneResult:
     group_points: int
     group_max_rows: int
     ms_per_build: float
-    table: List[Tuple[Tuple[int, int, int], float, float]]  # (candidate, ms, E_xc)
+    table: List[Tuple[Tuple[int, ...], float, float]]  # (candidate, ms, E_xc)
+    v_small_bf16: int = 0
+
+    @property
+    def shape(self) -> Tuple[int, int, int, int]:
+        return (self.block_points, self.group_points, self.group_max_rows, self.v_small_bf16)
 
     def engine(self, basis: BasisSet, grid: MolecularGrid, **kw) -> RksVxcEngine:
         return RksVxcEngine(basis, grid, block_points=self.block_points, group_points=self.group_points,
-                            group_max_rows=self.group_max_rows, **kw)
+                            group_max_rows=self.group_max_rows, v_small_bf16=self.v_small_bf16, **kw)
 
 
 def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, precision: str = "fp64",
-             candidates: Optional[Iterable[Tuple[int, int, int]]] = None, steps: int = 5, warmup: int = 1,
+             candidates: Optional[Iterable[Tuple[int, ...]]] = None, steps: int = 5, warmup: int = 1,
              e_tol: Optional[float] = None, keep_first_within: float = 0.0) -> TuneResult:
     """e_tol: largest accepted |E_xc - E_xc(first candidate)|; default 1e-9 Eh for
     FP64 builds, 1e-7 Eh for mixed ones (their rounding depends on the blocking).
@@ -61,8 +67,10 @@ def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, pre
     den = hr.DeviceArray(16)
     table = []
     e_first = None
-    for bp, gp, gr in cands:
-        eng = RksVxcEngine(basis, grid, block_points=bp, group_points=gp, group_max_rows=gr)
+    for cand in cands:
+        bp, gp, gr = cand[:3]
+        vs = cand[3] if len(cand) > 3 else 0
+        eng = RksVxcEngine(basis, grid, block_points=bp, group_points=gp, group_max_rows=gr, v_small_bf16=vs)
         for _ in range(max(warmup, 1)):
             eng.build_device(func, dP.address, dV.address, den.address, stream.address, precision)
         stream.synchronize()
@@ -77,9 +85,9 @@ def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, pre
         eng.close()
         if e_first is None:
             e_first = e
-        table.append(((bp, gp, gr), ms, e))
+        table.append(((bp, gp, gr, vs), ms, e))
     ok = [t for t in table if abs(t[2] - e_first) <= e_tol]
-    (bp, gp, gr), ms, _ = min(ok, key=lambda t: t[1])
+    best, ms, _ = min(ok, key=lambda t: t[1])
     if ms >= table[0][1] * (1.0 - keep_first_within):
-        (bp, gp, gr), ms, _ = table[0]
-    return TuneResult(bp, gp, gr, ms, table)
+        best, ms, _ = table[0]
+    return TuneResult(best[0], best[1], best[2], ms, table, best[3])

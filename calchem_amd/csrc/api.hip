@@ -83,10 +83,14 @@ struct Plan {
     size_t lds_limit = 0;
     int n_group_list = 0, reduce_waves = 1;
     int reduce_segments = 1;  // workgroups per AO row in the V reduction (fewer rows than CUs x 8: more)
-    // The register-resident group list is two launches: groups of more than
+    // The register-resident group list is three launches: groups of more than
     // kSmallGroupTiles tiles first (group_max_tiles kernel), then the rest
-    // (kSmallGroupTiles kernel), each costliest first.
+    // (kSmallGroupTiles kernel) in two lists, those whose shells and
+    // primitives keep the mixed kernel's LDS above half a CU's (one workgroup
+    // per CU) and then the others (two per CU), each costliest first.
     int n_group_big = 0, max_small_nsh = 0, max_small_nprim = 0, max_big_nsh = 0, max_big_nprim = 0;
+    int n_group_heavy = 0, max_heavy_nsh = 0, max_heavy_nprim = 0;
+    bool v_small_bf16 = false;  // mixed small-group V kernel in the BF16 three-term form
     int group_max_tiles = kGroupMaxTiles;
     int n_mol = 1, max_mol_ao = 0, n_xc_wg = 0;
     long long p_elems = 0;  // sum over molecules of n_ao^2 (packed P / V)
@@ -316,6 +320,9 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     if (want_rows != 0 && want_rows != 192 && want_rows != 272) return kBadArgs;
     const int aoc_mode = opt ? opt->ao_cache : 0;
     if (aoc_mode < 0 || aoc_mode > 2) return kBadArgs;
+    const int vform = opt ? opt->v_small_bf16 : 0;
+    if (vform < 0 || vform > 2) return kBadArgs;
+    P.v_small_bf16 = vform == 2;
     P.group_max_tiles = want_rows == 192 ? 12 : kGroupMaxTiles;
 
     // Device shell / AO tables.
@@ -675,11 +682,22 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
         return cost(x) > cost(y);
     });
     std::stable_partition(hglist.begin(), hglist.end(), [&](int g) { return hg[g].n_tiles > kSmallGroupTiles; });
+    int lds_max = 0;
+    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device) != hipSuccess)
+        lds_max = 65536;
+    P.lds_limit = static_cast<size_t>(std::max(lds_max, 65536));
+    const auto heavy = [&](int g) {
+        return hg[g].n_tiles <= kSmallGroupTiles &&
+               vgroup_lds_bytes(true, kSmallGroupTiles, hg[g].n_sh, hg[g].n_prim, P.v_small_bf16) > P.lds_limit / 2;
+    };
+    std::stable_partition(hglist.begin(), hglist.end(),
+                          [&](int g) { return hg[g].n_tiles > kSmallGroupTiles || heavy(g); });
     for (int g : hglist) {
-        const bool big = hg[g].n_tiles > kSmallGroupTiles;
+        const bool big = hg[g].n_tiles > kSmallGroupTiles, hv = heavy(g);
         P.n_group_big += big;
-        int& nsh = big ? P.max_big_nsh : P.max_small_nsh;
-        int& npr = big ? P.max_big_nprim : P.max_small_nprim;
+        P.n_group_heavy += hv;
+        int& nsh = big ? P.max_big_nsh : (hv ? P.max_heavy_nsh : P.max_small_nsh);
+        int& npr = big ? P.max_big_nprim : (hv ? P.max_heavy_nprim : P.max_small_nprim);
         nsh = std::max(nsh, hg[g].n_sh);
         npr = std::max(npr, hg[g].n_prim);
     }
@@ -705,24 +723,25 @@ int create_plan(const CalchemAoShell* sh, int n_shells, const CalchemAoPrim* pr,
     }
 
     // LDS budget per class and mode.
-    int lds_max = 0;
-    if (hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, P.device) != hipSuccess)
-        lds_max = 65536;
-    P.lds_limit = static_cast<size_t>(std::max(lds_max, 65536));
     for (int mode = 0; mode < 2; ++mode) {
         P.class_fits[mode].assign(P.classes.size(), 0);
         for (size_t c = 0; c < P.classes.size(); ++c)
             P.class_fits[mode][c] = rho_lds_bytes(mode, P.classes[c]) <= P.lds_limit;
     }
 
-    const int n_small = P.n_group_list - P.n_group_big;
+    const int n_small = P.n_group_list - P.n_group_big - P.n_group_heavy;
     if ((P.n_group_big > 0 &&
          vgroup_lds_bytes(false, P.group_max_tiles, P.max_big_nsh, P.max_big_nprim) > P.lds_limit) ||
+        (P.n_group_heavy > 0 &&
+         vgroup_lds_bytes(false, kSmallGroupTiles, P.max_heavy_nsh, P.max_heavy_nprim) > P.lds_limit) ||
         (n_small > 0 && vgroup_lds_bytes(false, kSmallGroupTiles, P.max_small_nsh, P.max_small_nprim) > P.lds_limit))
         return kLimit;
     P.v_fits_fp32 =
         (P.n_group_big == 0 || vgroup_lds_bytes(true, P.group_max_tiles, P.max_big_nsh, P.max_big_nprim) <= P.lds_limit) &&
-        (n_small == 0 || vgroup_lds_bytes(true, kSmallGroupTiles, P.max_small_nsh, P.max_small_nprim) <= P.lds_limit);
+        (P.n_group_heavy == 0 || vgroup_lds_bytes(true, kSmallGroupTiles, P.max_heavy_nsh, P.max_heavy_nprim,
+                                                  P.v_small_bf16) <= P.lds_limit) &&
+        (n_small == 0 || vgroup_lds_bytes(true, kSmallGroupTiles, P.max_small_nsh, P.max_small_nprim,
+                                          P.v_small_bf16) <= P.lds_limit);
     if (P.max_mpad_rho > 16384) return kLimit;  // the gather stages a block's AO indices in LDS
     P.reduce_waves = reduce_v_waves(P.max_mol_ao, P.lds_limit);
     if (P.reduce_waves < 1) return kLimit;
@@ -923,8 +942,12 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
         return kLaunchFail;
     VArgs vs = va;
     vs.group_list = va.group_list + P.n_group_big;
-    if (launch_vxc_group(vs, f32, kSmallGroupTiles, P.n_group_list - P.n_group_big, P.max_small_nsh,
-                         P.max_small_nprim, s) != hipSuccess)
+    if (launch_vxc_group(vs, f32, kSmallGroupTiles, P.n_group_heavy, P.max_heavy_nsh, P.max_heavy_nprim, s,
+                         P.v_small_bf16) != hipSuccess)
+        return kLaunchFail;
+    vs.group_list = va.group_list + P.n_group_big + P.n_group_heavy;
+    if (launch_vxc_group(vs, f32, kSmallGroupTiles, P.n_group_list - P.n_group_big - P.n_group_heavy,
+                         P.max_small_nsh, P.max_small_nprim, s, P.v_small_bf16) != hipSuccess)
         return kLaunchFail;
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);

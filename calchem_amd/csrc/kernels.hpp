@@ -229,9 +229,10 @@ hipError_t launch_xc_uks(const XcUksArgs& a, hipStream_t s);
 hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s);
 // Register-resident V kernel over groups of at most max_tiles x 16 rows
 // (max_tiles = 17 or 12).
-size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim);  // max_tiles: 8, 12 or 17
+// max_tiles: 8, 12 or 17; bf16: the mixed kernel's BF16 three-term operand form (always for 12 and 17)
+size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim, bool bf16 = true);
 hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
-                            hipStream_t s);
+                            hipStream_t s, bool bf16 = true);
 int reduce_v_waves(int max_mol_ao, size_t lds_limit);
 hipError_t launch_reduce_v(const ReduceArgs& a, int waves, int max_mol_ao, hipStream_t s);
 // Per molecule: (E_xc, N) = sums of its functional-stage partials, into out[2k], out[2k+1].

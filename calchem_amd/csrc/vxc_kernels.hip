@@ -1151,28 +1151,41 @@ __global__ __launch_bounds__((VGroupCfg<T, MAXT>::kWaves * 64)) void k_vxc_group
 // them is the only ordering needed; waves that finish their MFMA share early
 // go on with AO evaluation, so MFMA and VALU work overlap on the SIMDs.
 //
-// The tile-pair update Z_I P_J^T + P_I Z_J^T of a step is one 16 x 16 x 32
-// product over k = 2 p + s (point p, s = 0, 1): A[i][2p] = Z_I[i][p],
-// A[i][2p + 1] = P_I[i][p], B[2p][j] = P_J[j][p], B[2p + 1][j] = Z_J[j][p].
+// Two operand forms (template parameter BF), chosen per launch.  FP32: P
+// and Z stored as floats (row stride vgroup_ld), acc += Z_I P_J^T + P_I Z_J^T
+// on v_mfma_f32_16x16x4f32.  BF16 three-term: the tile-pair update
+// Z_I P_J^T + P_I Z_J^T of a step is one 16 x 16 x 32 product over
+// k = 2 q + s (slot q of the step's 16 points, s = 0, 1): A[i][2q] = Z_I[i][.],
+// A[i][2q + 1] = P_I[i][.], B[2q][j] = P_J[j][.], B[2q + 1][j] = Z_J[j][.] at
+// the point of slot q; slot 4 c + e holds point c + 4 e (any order of the
+// points is the same sum while A and B share it).
 // P and Z are formed in FP32 and split in three BF16 terms each
 // (split3_bf16), and the six products of terms whose indices sum to at most
 // 2 run on v_mfma_f32_16x16x32_bf16 with FP32 accumulation: FP32 operand
 // resolution at a sixth of the matrix-core cycles of the FP32 16x16x4 form
 // (6 x 16 against 8 x 32 cycles per pair and step).  LDS holds per buffer and
 // term one dword (P, Z) per (row, point): 16 dwords per row, the 16-byte chunk
-// of points 4c .. 4c + 3 stored at chunk c ^ ((row / 4) % 4), so that the 16
+// of slots 4c .. 4c + 3 stored at chunk c ^ ((row / 4) % 4), so that the 16
 // rows of an operand read fall on distinct bank groups.  A reads the same
-// dwords with the halves swapped.
+// dwords with the halves swapped.  The BF16 form saves matrix-core time and
+// costs the split (VALU) and 1.5x the operand LDS.  The 9-17-tile class
+// always runs it (20 waters: 152 -> 127 us per build; 64 clusters: 6.1 -> 5.1
+// ms per set); for the small-group class it is a plan option (the
+// autotuner's choice): faster on the 20-water cluster (0.593 -> 0.538 ms per
+// V stage), slower on the 64-cluster set (14.5 -> 19.5 ms), where the FP32
+// form, the default, is kept.
+
 // Point slots of the pipelined kernel: in the interval of step s it reads the
 // points of steps s + 1 and s + 2 and loads those of step s + 3, so three
 // slots suffice; wv is kept as the FP32 values the kernel uses.
 constexpr int kVPipeSlots = 3;
 
-template <int MAXT>
+template <int MAXT, bool BF>
 __host__ __device__ constexpr size_t vgroup_pipe_lds_fixed() {
     return (sizeof(double4) + sizeof(float4)) * kVPipeSlots * kVPts +
            sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1) +
-           sizeof(RowL) * MAXT * 16 + sizeof(unsigned) * 2 * 3 * MAXT * 16 * kVPts;
+           sizeof(RowL) * MAXT * 16 +
+           (BF ? sizeof(unsigned) * 2 * 3 * MAXT * 16 * kVPts : sizeof(float) * 4 * MAXT * 16 * vgroup_ld<float>());
 }
 
 // Waves of the pipelined kernel: 16 for groups of up to 12 or 17 tiles, 8
@@ -1183,12 +1196,13 @@ __host__ __device__ constexpr int vgroup_pipe_waves() {
     return MAXT <= 8 ? 8 : 16;
 }
 
-template <int MAXT>
+template <int MAXT, bool BF>
 __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pipe(VArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     using T = float;
     constexpr int NW = vgroup_pipe_waves<MAXT>(), NT = NW * 64, PPW = (vgroup_max_pairs<MAXT>() + NW - 1) / NW;
     static_assert(kVPts == 16, "one 16-dword row of (P, Z) per 16 points");
+    constexpr int LD = vgroup_ld<T>();  // FP32 form: row stride of P and Z
     const VGroup G = a.groups[a.group_list[blockIdx.x]];
     const int m = G.m, nsh = G.n_sh, npts = G.n_pts;
     const int T16 = (m + 15) >> 4, mp = T16 << 4;
@@ -1202,9 +1216,12 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
     q += sizeof(int2) * ((vgroup_max_pairs<MAXT>() + 1) & ~1);
     RowL* sRow = reinterpret_cast<RowL*>(q);
     q += sizeof(RowL) * MAXT * 16;
-    unsigned* sW = reinterpret_cast<unsigned*>(q);  // buffer b, term t at sW + (3 b + t) S: [row][16 points]
-    constexpr int S = MAXT * 16 * kVPts;
-    q += sizeof(unsigned) * 6 * S;
+    // BF16 form: buffer b, term t at sW + (3 b + t) S, [row][16 points];
+    // FP32 form: buffer b: phi at sPZ + 2 b SF, Z at sPZ + (2 b + 1) SF
+    unsigned* sW = reinterpret_cast<unsigned*>(q);
+    T* sPZ = reinterpret_cast<T*>(q);
+    constexpr int S = MAXT * 16 * kVPts, SF = MAXT * 16 * LD;
+    q += BF ? sizeof(unsigned) * 6 * S : sizeof(T) * 4 * SF;
     float2* sRad = reinterpret_cast<float2*>(q);  // buffer b at sRad + b nsh kVPts
     q += sizeof(float2) * 2 * nsh * kVPts;
     DevShell* sSh = reinterpret_cast<DevShell*>(q);
@@ -1226,7 +1243,7 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         int ti = 0;
         while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
         const int tj = p - ti * (ti + 1) / 2;
-        sPair[p] = make_int2(ti * 16 * kVPts, tj * 16 * kVPts);
+        sPair[p] = BF ? make_int2(ti * 16 * kVPts, tj * 16 * kVPts) : make_int2(ti * 16 * LD, tj * 16 * LD);
     }
 
     // points and wv of step st into its slot (zero past the group's points)
@@ -1263,10 +1280,35 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
         const double4* pt = sPt + (st % kVPipeSlots) * kVPts;
         const float4* wv = sWv + (st % kVPipeSlots) * kVPts;
         const float2* rad = sRad + (st & 1) * nsh * kVPts;
+        if constexpr (!BF) {
+            T* P = sPZ + 2 * (st & 1) * SF;
+            T* Z = P + SF;
+            // a thread takes one row at four points (p0, p0 + 4, p0 + 8, p0 + 12):
+            // the row's record, its decoding and its shell centre serve all four
+            for (int idx = tid; idx < mp * 4; idx += NT) {
+                const int r = idx >> 2, p0 = idx & 3;
+                // padding rows (shell -1, scale 0) read shell 0 and come out zero,
+                // and so do points past the group's end (zero (R0, A))
+                const RowL row = sRow[r];
+                const int sh = max(row.sh, 0);
+                const DevShell shl = sSh[sh];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int p = p0 + 4 * k;
+                    const double4 x = pt[p];
+                    const float2 pz = pz_from_radial_row_f(shl, row, x.x, x.y, x.z, rad[sh * kVPts + p], wv[p]);
+                    P[r * LD + p] = pz.x;
+                    Z[r * LD + p] = pz.y;
+                }
+            }
+            return;
+        }
         unsigned* W = sW + 3 * (st & 1) * S;
-        // a thread takes one row at the four points 4 c .. 4 c + 3: the row's
-        // record, its decoding and its shell centre serve all four, and each
-        // term's four dwords go to LDS as one 16-byte store
+        // a thread takes one row at the four points c, c + 4, c + 8, c + 12
+        // (slots 4 c .. 4 c + 3; the threads of a row read four consecutive
+        // points at a time, free of LDS bank conflicts): the row's record, its
+        // decoding and its shell centre serve all four, and each term's four
+        // dwords go to LDS as one 16-byte store
         for (int idx = tid; idx < mp * 4; idx += NT) {
             const int r = idx >> 2, c = idx & 3;
             // padding rows (shell -1, scale 0) read shell 0 and come out zero,
@@ -1277,7 +1319,7 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
             v4u t[3];
 #pragma unroll
             for (int k = 0; k < 4; ++k) {
-                const int p = 4 * c + k;
+                const int p = c + 4 * k;
                 const double4 x = pt[p];
                 const float4 w = wv[p];
                 const float2 pz = pz_from_radial_row_f(shl, row, x.x, x.y, x.z, rad[sh * kVPts + p], w);
@@ -1294,9 +1336,37 @@ __global__ __launch_bounds__(vgroup_pipe_waves<MAXT>() * 64) void k_vxc_group_pi
     typename M::acc acc[PPW];
 #pragma unroll
     for (int w = 0; w < PPW; ++w) acc[w] = M::zero();
-    // this lane's 16-byte chunk of a tile row: points 4 kq .. 4 kq + 3 of row col
+    // this lane's 16-byte chunk of a tile row: slots 4 kq .. 4 kq + 3 of row col
     const int lane_at = col * kVPts + 4 * (kq ^ ((col >> 2) & 3));
     auto update = [&](int st) {
+        if constexpr (!BF) {
+            using V4 = typename Vec4<T>::type;
+            const T* P = sPZ + 2 * (st & 1) * SF;
+            const T* Z = P + SF;
+            int cur_i = -1;
+            V4 zi, pi;
+#pragma unroll
+            for (int w = 0; w < PPW; ++w) {
+                const int p = wave * per_wave + w;
+                if (w >= per_wave || p >= npairs) continue;
+                const int2 pr = sPair[p];
+                if (w == 0 || pr.x != cur_i) {
+                    cur_i = pr.x;
+                    const int ri = pr.x + col * LD + 4 * kq;
+                    zi = lds_load4(Z + ri);
+                    pi = lds_load4(P + ri);
+                }
+                const int rj = pr.y + col * LD + 4 * kq;
+                const V4 pj = lds_load4(P + rj);
+                const V4 zj = lds_load4(Z + rj);
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    acc[w] = M::op(zi[k], pj[k], acc[w]);
+                    acc[w] = M::op(pi[k], zj[k], acc[w]);
+                }
+            }
+            return;
+        }
         const unsigned* W = sW + 3 * (st & 1) * S;
         int cur_i = -1;
         v4u a[3];
@@ -1706,9 +1776,9 @@ hipError_t launch_vxc(const VArgs& a, bool fp32, int n_pairs, hipStream_t s) {
 }
 
 template <typename T, int MAXT>
-static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim) {
+static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim, bool bf16) {
     if constexpr (sizeof(T) == 4) {  // k_vxc_group_pipe
-        return vgroup_pipe_lds_fixed<MAXT>() + sizeof(float2) * 2 * (size_t)max_nsh * kVPts +
+        return (bf16 ? vgroup_pipe_lds_fixed<MAXT, true>() : vgroup_pipe_lds_fixed<MAXT, false>()) + sizeof(float2) * 2 * (size_t)max_nsh * kVPts +
                sizeof(DevShell) * (size_t)max_nsh + sizeof(double2) * (size_t)max_prim;
     } else {
         const size_t rad = sizeof(typename VGroupCfg<T, MAXT>::Rad) * (size_t)((max_nsh * kVPts + 1) & ~1);
@@ -1717,15 +1787,18 @@ static size_t vgroup_lds_bytes_t(int max_nsh, int max_prim) {
     }
 }
 
-size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim) {
+size_t vgroup_lds_bytes(bool fp32, int max_tiles, int max_nsh, int max_prim, bool bf16) {
     if (max_tiles == 8)
-        return fp32 ? vgroup_lds_bytes_t<float, 8>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 8>(max_nsh, max_prim);
+        return fp32 ? vgroup_lds_bytes_t<float, 8>(max_nsh, max_prim, bf16)
+                    : vgroup_lds_bytes_t<double, 8>(max_nsh, max_prim, false);
     if (max_tiles == 12)
-        return fp32 ? vgroup_lds_bytes_t<float, 12>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 12>(max_nsh, max_prim);
-    return fp32 ? vgroup_lds_bytes_t<float, 17>(max_nsh, max_prim) : vgroup_lds_bytes_t<double, 17>(max_nsh, max_prim);
+        return fp32 ? vgroup_lds_bytes_t<float, 12>(max_nsh, max_prim, bf16)
+                    : vgroup_lds_bytes_t<double, 12>(max_nsh, max_prim, false);
+    return fp32 ? vgroup_lds_bytes_t<float, 17>(max_nsh, max_prim, bf16)
+                : vgroup_lds_bytes_t<double, 17>(max_nsh, max_prim, false);
 }
 
-template <typename T, int MAXT>
+template <typename T, int MAXT, bool BF = false>
 static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, int max_prim, hipStream_t s) {
     static int attr_set = 0;
     if (!attr_set) {
@@ -1734,7 +1807,7 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
         if (e == hipSuccess) e = hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         const void* fn;
         if constexpr (sizeof(T) == 4)
-            fn = reinterpret_cast<const void*>(&k_vxc_group_pipe<MAXT>);
+            fn = reinterpret_cast<const void*>(&k_vxc_group_pipe<MAXT, BF>);
         else
             fn = reinterpret_cast<const void*>(&k_vxc_group<T, MAXT>);
         if (e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, lim);
@@ -1742,9 +1815,9 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
         attr_set = 1;
     }
     if (n_groups <= 0) return hipSuccess;
-    const size_t lds = vgroup_lds_bytes_t<T, MAXT>(max_nsh, max_prim);
+    const size_t lds = vgroup_lds_bytes_t<T, MAXT>(max_nsh, max_prim, BF);
     if constexpr (sizeof(T) == 4) {
-        hipLaunchKernelGGL((k_vxc_group_pipe<MAXT>), dim3(n_groups), dim3(vgroup_pipe_waves<MAXT>() * 64), lds, s,
+        hipLaunchKernelGGL((k_vxc_group_pipe<MAXT, BF>), dim3(n_groups), dim3(vgroup_pipe_waves<MAXT>() * 64), lds, s,
                            a);
     } else {
         constexpr int threads = VGroupCfg<T, MAXT>::kWaves * 64;
@@ -1754,14 +1827,16 @@ static hipError_t launch_vxc_group_t(const VArgs& a, int n_groups, int max_nsh, 
 }
 
 hipError_t launch_vxc_group(const VArgs& a, bool fp32, int max_tiles, int n_groups, int max_nsh, int max_prim,
-                            hipStream_t s) {
-    if (max_tiles == 8)
-        return fp32 ? launch_vxc_group_t<float, 8>(a, n_groups, max_nsh, max_prim, s)
-                    : launch_vxc_group_t<double, 8>(a, n_groups, max_nsh, max_prim, s);
+                            hipStream_t s, bool bf16) {
+    if (max_tiles == 8) {
+        if (!fp32) return launch_vxc_group_t<double, 8>(a, n_groups, max_nsh, max_prim, s);
+        return bf16 ? launch_vxc_group_t<float, 8, true>(a, n_groups, max_nsh, max_prim, s)
+                    : launch_vxc_group_t<float, 8, false>(a, n_groups, max_nsh, max_prim, s);
+    }
     if (max_tiles == 12)
-        return fp32 ? launch_vxc_group_t<float, 12>(a, n_groups, max_nsh, max_prim, s)
+        return fp32 ? launch_vxc_group_t<float, 12, true>(a, n_groups, max_nsh, max_prim, s)
                     : launch_vxc_group_t<double, 12>(a, n_groups, max_nsh, max_prim, s);
-    return fp32 ? launch_vxc_group_t<float, 17>(a, n_groups, max_nsh, max_prim, s)
+    return fp32 ? launch_vxc_group_t<float, 17, true>(a, n_groups, max_nsh, max_prim, s)
                 : launch_vxc_group_t<double, 17>(a, n_groups, max_nsh, max_prim, s);
 }
 

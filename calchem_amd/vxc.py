@@ -41,7 +41,7 @@ def _records(basis: BasisSet):
 class RksVxcEngine:
     def __init__(self, basis: BasisSet, grid: MolecularGrid, block_points: int = 0,
                  group_points: int = 0, memory_limit_bytes: int = 0, mixed_exact_threshold: float = 0.0,
-                 rho_tile_points: int = 0, group_max_rows: int = 0, ao_cache: int = 0):
+                 rho_tile_points: int = 0, group_max_rows: int = 0, ao_cache: int = 0, v_small_bf16: int = 0):
         self._lib = _native.require_device()
         self.n_ao = basis.n_ao
         self.n_points = grid.n_points
@@ -50,7 +50,7 @@ class RksVxcEngine:
         shells, prims, comps = _records(basis)
         xyzw = grid.xyzw()
         opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold,
-                                  rho_tile_points, group_max_rows, ao_cache)
+                                  rho_tile_points, group_max_rows, ao_cache, v_small_bf16)
         h = _native.new_handle()
         rc = self._lib.calchem_hip_vxc_create(
             _native.ptr(shells), len(shells), _native.ptr(prims), len(prims), _native.ptr(comps),
@@ -226,7 +226,7 @@ class BatchedVxcEngine(RksVxcEngine):
 
     def __init__(self, systems, block_points: int = 0, group_points: int = 0, memory_limit_bytes: int = 0,
                  mixed_exact_threshold: float = 0.0, rho_tile_points: int = 0, group_max_rows: int = 0,
-                 ao_cache: int = 0):
+                 ao_cache: int = 0, v_small_bf16: int = 0):
         self._lib = _native.require_device()
         self.handle = 0
         self._kernel_invocations = 0
@@ -260,7 +260,7 @@ class BatchedVxcEngine(RksVxcEngine):
         pb = np.asarray(point_begin, dtype=np.int32)
         na = np.asarray(n_ao, dtype=np.int32)
         opts = _native.VxcOptions(block_points, group_points, memory_limit_bytes, mixed_exact_threshold,
-                                  rho_tile_points, group_max_rows, ao_cache)
+                                  rho_tile_points, group_max_rows, ao_cache, v_small_bf16)
         h = _native.new_handle()
         rc = self._lib.calchem_hip_vxc_create_batched(self.n_mol, _native.ptr(sb), _native.ptr(pb), _native.ptr(na),
                                                       _native.ptr(shells), _native.ptr(prims), len(prims),

@@ -86,7 +86,13 @@ typedef struct {
      * free device memory (and memory_limit_bytes), 1 = never, 2 = always (-3
      * / -6 when it cannot be had).  FP64 builds never use it. */
     int32_t ao_cache;
-    int32_t reserved;
+    /* Operand form of the mixed build's V-stage kernel for groups of up to 8
+     * 16-row tiles: 0 or 1 = FP32 operands on v_mfma_f32_16x16x4f32 (the
+     * default), 2 = each FP32 operand split into three BF16 terms on
+     * v_mfma_f32_16x16x32_bf16 (the form the larger groups always use); which
+     * is faster depends on the workload, so it is an autotuner choice.  Other
+     * values: -1. */
+    int32_t v_small_bf16;
 } CalchemVxcOptions;
 
 typedef struct {

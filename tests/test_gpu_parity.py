@@ -131,6 +131,29 @@ def test_fused_fp32_c8h18_error_report(func):
     assert np.abs(V32 - V32.T).max() == 0.0
 
 
+def test_small_group_v_operand_forms_agree():
+    """The mixed build's small-group V kernel in its two operand forms (FP32,
+    and three BF16 terms per FP32 operand): E_xc and N come from the rho stage
+    and the functional, so they are bitwise equal; V differs at FP32 rounding,
+    and both stay within the mixed build's bar of the FP64 build."""
+    mol, basis, P, w = _case("c8h18")
+    grid = MolecularGrid.explicit(mol, w["n_rad"], w["n_ang"])
+    e64, V64, _ = RksVxcEngine(basis, grid).evaluate_full("b3lyp", P, "fp64")
+    out = {}
+    for form in (1, 2):
+        eng = RksVxcEngine(basis, grid, v_small_bf16=form)
+        out[form] = eng.evaluate_full("b3lyp", P, "fp32")
+        eng.close()
+    (e1, V1, n1), (e2, V2, n2) = out[1], out[2]
+    d12, d1, d2 = np.abs(V1 - V2).max(), np.abs(V1 - V64).max(), np.abs(V2 - V64).max()
+    print(f"small-group V forms: max|V_fp32 - V_bf16x3| {d12:.2e}; against FP64 {d1:.2e} / {d2:.2e}")
+    assert e1 == e2 and n1 == n2
+    assert d12 < 2e-6 and d1 < 1e-4 and d2 < 1e-4
+    assert np.abs(V2 - V2.T).max() == 0.0
+    with pytest.raises(RuntimeError):
+        RksVxcEngine(basis, grid, v_small_bf16=3)
+
+
 @pytest.mark.parametrize("block_points,group_points", [(64, 256), (96, 512), (512, 4096)])
 def test_fused_fp64_c8h18_block_shapes(block_points, group_points):
     """Other block / group sizes give the same result."""
