@@ -183,14 +183,15 @@ def load_valu(workload: str):
 RHO_STAGE_KERNELS = {"fp64": ("k_rho<double, false, 1, 0>",),
                      "fp32": ("k_rho_aoc2", "k_rho<float, true, 1, 2>")}
 # Kernels of the V stage (one launch per group class).
-V_STAGE_KERNELS = {"fp64": ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>"),
-                   "fp32": ("k_vxc_group_pipe<17>", "k_vxc_group_pipe<8>")}
+V_FP32_KERNELS = ("k_vxc_group_pipe<17, true>", "k_vxc_group_pipe<12, true>", "k_vxc_group_pipe<8, false>",
+                  "k_vxc_group_pipe<8, true>")
+V_STAGE_KERNELS = {"fp64": ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>"), "fp32": V_FP32_KERNELS}
 # Stages whose HBM traffic per build is digested, by the kernel name the roofline reports.
 TRAFFIC_STAGES = {"k_rho": RHO_STAGE_KERNELS, "k_vxc_group": V_STAGE_KERNELS}
 VALU_STAGES = {
     "ao_evaluation_fp64_rho": ("rho", "fp64", ("k_rho<double, false, 1, 0>",)),
     "functional": ("xc", "fp64", ("k_xc<3>",)),
-    "ao_evaluation_fp32_v": ("vxc", "fp32", ("k_vxc_group_pipe<17>", "k_vxc_group_pipe<8>")),
+    "ao_evaluation_fp32_v": ("vxc", "fp32", V_FP32_KERNELS),
     "v_fp64": ("vxc", "fp64", ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>")),
     "rho_fp32_from_cache": ("rho", "fp32", ("k_rho_aoc2", "k_rho<float, true, 1, 2>")),
 }

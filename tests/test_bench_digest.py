@@ -46,10 +46,10 @@ def test_stage_totals_and_copy_calibration(tmp_path, monkeypatch):
                   ("chx::k_rho_aoc2(chx::RhoArgs)", 256, "WRITE_SIZE", 10.0),
                   ("void chx::k_rho<double, false, 1, 0>(chx::RhoArgs)", 512, "WRITE_SIZE", 5.0)]
         # V stage: two group kernels launched on the same grid size, one launch each per build
-        fetch += [("void chx::k_vxc_group_pipe<17>(chx::VArgs)", 1024, "FETCH_SIZE", 700.0),
-                  ("void chx::k_vxc_group_pipe<8>(chx::VArgs)", 1024, "FETCH_SIZE", 300.0)]
-        write += [("void chx::k_vxc_group_pipe<17>(chx::VArgs)", 1024, "WRITE_SIZE", 70.0),
-                  ("void chx::k_vxc_group_pipe<8>(chx::VArgs)", 1024, "WRITE_SIZE", 30.0)]
+        fetch += [("void chx::k_vxc_group_pipe<17, true>(chx::VArgs)", 1024, "FETCH_SIZE", 700.0),
+                  ("void chx::k_vxc_group_pipe<8, false>(chx::VArgs)", 1024, "FETCH_SIZE", 300.0)]
+        write += [("void chx::k_vxc_group_pipe<17, true>(chx::VArgs)", 1024, "WRITE_SIZE", 70.0),
+                  ("void chx::k_vxc_group_pipe<8, false>(chx::VArgs)", 1024, "WRITE_SIZE", 30.0)]
     _write(str(tmp_path / "mf"), fetch)
     _write(str(tmp_path / "mw"), write)
     bench.digest_traffic(str(tmp_path / "mf"), str(tmp_path / "mw"), "water20", "test")
