@@ -503,21 +503,23 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho(RhoArgs a) {
     }
 }
 
-// Mixed rho stage from the AO cache, one spin, with the workgroup's two
-// 16-point sub-tiles contracted together: every density chunk of
-// T = P_sub Phi that a wave loads feeds the MFMAs of both sub-tiles (32
-// points), so the density blocks are read half as often as by one sub-tile
-// at a time.  LDS holds the B operands of both sub-tiles: Phi of the FP64
-// leading rows (below m64) and of the FP32 rows (from m64 on).  Wave w takes
-// the 16-row tiles w, w + 4, ...; after each tile's contraction the epilogue
-// reads the tile's gradient slots of one sub-tile, then the other, from the
-// cache's gradient plane and phi from LDS, so the cache is read once.
-// (Requesting the gradient slots before the contraction was slower: 0.570
-// against 0.549 ms per 20-water stage; overlaying the sums on the operands,
-// for less LDS per workgroup, made no difference.  In the two-spin
-// k_rho<float, true, 2, kAocLoad>, phi from LDS made the unrestricted build
-// slower, 3.22 against 3.15 ms, so that kernel reads it from the cache.)  The sums are those of
-// k_rho<float, true, 1, kAocLoad>, formed in the same order.
+// Mixed rho stage from the AO cache, NS spins (1, or 2 for unrestricted
+// builds, whose two density blocks share one read of the cache), with the
+// workgroup's two 16-point sub-tiles contracted together: every density
+// chunk of T = P_sub Phi that a wave loads feeds the MFMAs of both sub-tiles
+// (32 points), so the density blocks are read half as often as by one
+// sub-tile at a time.  LDS holds the B operands of both sub-tiles: Phi of the
+// FP64 leading rows (below m64) and of the FP32 rows (from m64 on).  Wave w
+// takes the 16-row tiles w, w + 4, ...; after each tile's contraction (per
+// spin) the epilogue reads the tile's gradient slots of one sub-tile, then
+// the other, from the cache's gradient plane and phi from LDS, so the cache
+// is read once.  (Requesting the gradient slots before the contraction was
+// slower: 0.570 against 0.549 ms per 20-water stage; overlaying the sums on
+// the operands, for less LDS per workgroup, made no difference.)  The sums of
+// each spin are those of k_rho<float, true, 1, kAocLoad>, formed in the same
+// order; with NS = 2 the unrestricted 20-water build took 2.54 against 3.01
+// ms with k_rho<float, true, 2, kAocLoad>, bitwise equal.
+template <int NS>
 __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int PTS = kRhoPts, PT2 = 2 * kRhoPts;
@@ -528,16 +530,18 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     const int mp = blk.m_pad, T16 = mp >> 4, m64 = blk.m64, L16 = m64 >> 4, mr = mp - m64;
     const int np_all = min(PT2, blk.n_pts - tile.y);
     const bool two = np_all > PTS;  // the second sub-tile exists (and has its cache slots)
-    double* sRed = reinterpret_cast<double*>(smem);                   // [4 waves][4 sums][PT2]
-    double* sPhi64 = sRed + 16 * PT2;                                  // [2 sub-tiles][m64][PTS]
+    double* sRed = reinterpret_cast<double*>(smem);                   // [4 waves][NS][4 sums][PT2]
+    double* sPhi64 = sRed + 16 * NS * PT2;                             // [2 sub-tiles][m64][PTS]
     float* sPhi = reinterpret_cast<float*>(sPhi64 + 2 * m64 * PTS);    // [2 sub-tiles][mr][PTS]
 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int col = lane & 15, kq = lane >> 4;
-    // the block's density blocks: uniform bases, 32-bit per-lane offsets
-    const float* __restrict__ PSb = static_cast<const float*>(a.PS) + blk.psub;
-    const double* __restrict__ P64b = a.P64 + blk.psub;
+    // the block's density blocks (per spin): uniform bases, 32-bit per-lane offsets
+    const float* __restrict__ PS0 = static_cast<const float*>(a.PS) + blk.psub;
+    const double* __restrict__ P640 = a.P64 + blk.psub;
+    const float* __restrict__ PS1 = NS == 2 ? static_cast<const float*>(a.PSb) + blk.psub : PS0;
+    const double* __restrict__ P641 = NS == 2 ? a.P64b + blk.psub : P640;
     // Slot v of row tile it of sub-tile s in the AO cache: its phi and its
     // gradient (see aoc_sub_bytes).  A tile without a second sub-tile reads
     // the first one in its place: that sub-tile's sums are then never written.
@@ -580,22 +584,26 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     }
     __syncthreads();
 
-    // sums[s] = (rho, grad rho / 2) at this lane's point of sub-tile s
-    double sums[2][4];
+    // sums[sp][s] = (rho, grad rho / 2) of spin sp at this lane's point of sub-tile s
+    double sums[NS][2][4];
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
+    for (int sp = 0; sp < NS; ++sp)
 #pragma unroll
-        for (int k = 0; k < 4; ++k) sums[s][k] = 0.0;
-    auto accumulate = [&](int s, const v4f& acc, const v4d& acc64, int v, const AoVal& f) {
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sums[sp][s][k] = 0.0;
+    auto accumulate = [&](int sp, int s, const v4f& acc, const v4d& acc64, int v, const AoVal& f) {
         const double tv = static_cast<double>(acc[v]) + acc64[v];
-        sums[s][0] += f.v * tv;
-        sums[s][1] += f.gx * tv;
-        sums[s][2] += f.gy * tv;
-        sums[s][3] += f.gz * tv;
+        sums[sp][s][0] += f.v * tv;
+        sums[sp][s][1] += f.gx * tv;
+        sums[sp][s][2] += f.gy * tv;
+        sums[sp][s][3] += f.gz * tv;
     };
     // T of row tile it for both sub-tiles: this lane ends up with
     // T[4 kq + v][col] of sub-tile s in acc[s][v] + acc64[s][v].
-    auto contract = [&](int it, v4f (&acc)[2], v4d (&acc64)[2]) {
+    auto contract = [&](int it, v4f (&acc)[2], v4d (&acc64)[2], int sp) {
+        const float* __restrict__ PSb = sp ? PS1 : PS0;
+        const double* __restrict__ P64b = sp ? P641 : P640;
         const int i0 = it << 4;
         acc[0] = acc[1] = Mma<float>::zero();
         acc64[0] = acc64[1] = Mma<double>::zero();
@@ -657,11 +665,12 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
         if (c + 1 < nch) chunk(r1, c + 1);
     };
     for (int it = wave; it < T16; it += 4) {
-        v4f acc[2];
-        v4d acc64[2];
+        v4f acc[NS][2];
+        v4d acc64[NS][2];
         if (it < L16) {
-            // leading row tile: contraction first, then its 32-byte slots
-            contract(it, acc, acc64);
+            // leading row tile: its 32-byte slots after the contraction
+#pragma unroll
+            for (int sp = 0; sp < NS; ++sp) contract(it, acc[sp], acc64[sp], sp);
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 AoVal w[4];
@@ -671,11 +680,14 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
                     w[v] = AoVal{sPhi64[(s * m64 + (it << 4) + 4 * kq + v) * PTS + col], g[0], g[1], g[2]};
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v) accumulate(s, acc[s], acc64[s], v, w[v]);
+                for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) accumulate(sp, s, acc[sp][s], acc64[sp][s], v, w[v]);
                 __builtin_amdgcn_sched_barrier(0);
             }
         } else {
-            contract(it, acc, acc64);
+#pragma unroll
+            for (int sp = 0; sp < NS; ++sp) contract(it, acc[sp], acc64[sp], sp);
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 float4 w[4];
@@ -685,8 +697,10 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
                     w[v] = make_float4(sPhi[(s * mr + (it << 4) - m64 + 4 * kq + v) * PTS + col], g[0], g[1], g[2]);
                 }
 #pragma unroll
-                for (int v = 0; v < 4; ++v)
-                    accumulate(s, acc[s], acc64[s], v, AoVal{w[v].x, w[v].y, w[v].z, w[v].w});
+                for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        accumulate(sp, s, acc[sp][s], acc64[sp][s], v, AoVal{w[v].x, w[v].y, w[v].z, w[v].w});
                 __builtin_amdgcn_sched_barrier(0);
             }
         }
@@ -694,31 +708,41 @@ __global__ __launch_bounds__(kThreads, 2) void k_rho_aoc2(RhoArgs a) {
     // Sum over the four row groups of the wave, then over the waves, in the
     // order k_rho uses.
 #pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sums[s][k] += __shfl_xor(sums[s][k], 16);
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) sums[s][k] += __shfl_xor(sums[s][k], 32);
-    if (kq == 0) {
+    for (int sp = 0; sp < NS; ++sp)
 #pragma unroll
         for (int s = 0; s < 2; ++s)
 #pragma unroll
-            for (int k = 0; k < 4; ++k) sRed[(wave * 4 + k) * PT2 + s * PTS + col] = sums[s][k];
+            for (int k = 0; k < 4; ++k) sums[sp][s][k] += __shfl_xor(sums[sp][s][k], 16);
+#pragma unroll
+    for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+        for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sums[sp][s][k] += __shfl_xor(sums[sp][s][k], 32);
+    if (kq == 0) {
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp)
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) sRed[((wave * NS + sp) * 4 + k) * PT2 + s * PTS + col] = sums[sp][s][k];
     }
     __syncthreads();
     if (tid < np_all) {
-        double r[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) r[k] = sRed[k * PT2 + tid];
-#pragma unroll
-        for (int w = 1; w < 4; ++w)
-#pragma unroll
-            for (int k = 0; k < 4; ++k) r[k] += sRed[(w * 4 + k) * PT2 + tid];
         const int g = blk.pt_begin + tile.y + tid;
         const int go = a.rho_perm ? a.perm[g] : g;
-        a.rho_out[go] = make_double4(r[0], 2.0 * r[1], 2.0 * r[2], 2.0 * r[3]);
+        double4* const outs[2] = {a.rho_out, a.rho_out_b};
+#pragma unroll
+        for (int sp = 0; sp < NS; ++sp) {
+            double r[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) r[k] = sRed[(sp * 4 + k) * PT2 + tid];
+#pragma unroll
+            for (int w = 1; w < 4; ++w)
+#pragma unroll
+                for (int k = 0; k < 4; ++k) r[k] += sRed[((w * NS + sp) * 4 + k) * PT2 + tid];
+            outs[sp][go] = make_double4(r[0], 2.0 * r[1], 2.0 * r[2], 2.0 * r[3]);
+        }
     }
 }
 
@@ -1666,28 +1690,29 @@ static hipError_t launch_rho_t(const RhoArgs& a, const RhoClass& c, hipStream_t 
     return hipGetLastError();
 }
 
-// LDS of k_rho_aoc2: the sums, then the B operands of both sub-tiles.
-static size_t rho_aoc2_lds_bytes(const RhoClass& c) {
-    return sizeof(double) * 16 * 2 * kRhoPts + 2 * kRhoPts * (sizeof(double) * (size_t)c.max_m64 +
-                                                               sizeof(float) * (size_t)c.max_mpad);
+// LDS of k_rho_aoc2<NS>: the sums, then the B operands of both sub-tiles.
+static size_t rho_aoc2_lds_bytes(const RhoClass& c, int ns) {
+    return sizeof(double) * 16 * (size_t)ns * 2 * kRhoPts +
+           2 * kRhoPts * (sizeof(double) * (size_t)c.max_m64 + sizeof(float) * (size_t)c.max_mpad);
 }
 
-// The mixed rho stage from the AO cache: k_rho_aoc2 where its LDS fits,
-// else k_rho<float, true, 1, kAocLoad> (one sub-tile at a time, less LDS).
+// The mixed rho stage from the AO cache: k_rho_aoc2<NS> where its LDS fits,
+// else k_rho<float, true, NS, kAocLoad> (one sub-tile at a time, less LDS).
+template <int NS>
 static hipError_t launch_rho_aoc2(const RhoArgs& a, const RhoClass& c, hipStream_t s) {
-    const size_t lds = rho_aoc2_lds_bytes(c);
+    const size_t lds = rho_aoc2_lds_bytes(c, NS);
     static int lim = 0;
     if (lim == 0) {
         int dev = 0, l = 0;
         hipError_t e = hipGetDevice(&dev);
         if (e == hipSuccess) e = hipDeviceGetAttribute(&l, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
         if (e == hipSuccess)
-            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho_aoc2),
+            e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_rho_aoc2<NS>),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, l);
         if (e != hipSuccess) return e;
         lim = l;
     }
-    if (lds > static_cast<size_t>(lim)) return launch_rho_t<float, true, 1, kAocLoad>(a, c, s);
+    if (lds > static_cast<size_t>(lim)) return launch_rho_t<float, true, NS, kAocLoad>(a, c, s);
     if (c.n_tiles <= 0) return hipSuccess;
     RhoArgs b = a;
     b.tiles = a.tiles + c.tile_begin;
@@ -1698,7 +1723,7 @@ static hipError_t launch_rho_aoc2(const RhoArgs& a, const RhoClass& c, hipStream
         longest = std::max(longest, c.seg[x + 1] - c.seg[x]);
     }
     b.seg[kXcds] = c.seg[kXcds];
-    hipLaunchKernelGGL(k_rho_aoc2, dim3(kXcds * longest), dim3(kThreads), lds, s, b);
+    hipLaunchKernelGGL(k_rho_aoc2<NS>, dim3(kXcds * longest), dim3(kThreads), lds, s, b);
     return hipGetLastError();
 }
 
@@ -1718,7 +1743,7 @@ hipError_t launch_rho(const RhoArgs& a, int mode, const RhoClass& c, hipStream_t
     if (mode == kRhoFp64)
         return ns == 2 ? launch_rho_t<double, false, 2>(a, c, s) : launch_rho_t<double, false, 1>(a, c, s);
     if (a.aoc)
-        return ns == 2 ? launch_rho_t<float, true, 2, kAocLoad>(a, c, s) : launch_rho_aoc2(a, c, s);
+        return ns == 2 ? launch_rho_aoc2<2>(a, c, s) : launch_rho_aoc2<1>(a, c, s);
     return ns == 2 ? launch_rho_t<float, true, 2>(a, c, s) : launch_rho_t<float, true, 1>(a, c, s);
 }
 
