@@ -178,10 +178,10 @@ def load_valu(workload: str):
 # --------------------------------------------------------------------------
 # Kernels of each rho stage (single spin) and of the VALU-roofline stages, by
 # rocprofv3 kernel name.  The mixed rho stage from the AO cache is
-# k_rho_aoc2 (k_rho<float, true, 1, 2> for a launch class whose LDS does not
-# fit it).
+# k_rho_aoc2<1> (k_rho<float, true, 1, 2> for a launch class whose LDS does
+# not fit it).
 RHO_STAGE_KERNELS = {"fp64": ("k_rho<double, false, 1, 0>",),
-                     "fp32": ("k_rho_aoc2", "k_rho<float, true, 1, 2>")}
+                     "fp32": ("k_rho_aoc2<1>", "k_rho<float, true, 1, 2>")}
 # Kernels of the V stage (one launch per group class).
 V_FP32_KERNELS = ("k_vxc_group_pipe<17, true>", "k_vxc_group_pipe<12, true>", "k_vxc_group_pipe<8, false>",
                   "k_vxc_group_pipe<8, true>")
@@ -193,7 +193,7 @@ VALU_STAGES = {
     "functional": ("xc", "fp64", ("k_xc<3>",)),
     "ao_evaluation_fp32_v": ("vxc", "fp32", V_FP32_KERNELS),
     "v_fp64": ("vxc", "fp64", ("k_vxc_group<double, 17>", "k_vxc_group<double, 8>")),
-    "rho_fp32_from_cache": ("rho", "fp32", ("k_rho_aoc2", "k_rho<float, true, 1, 2>")),
+    "rho_fp32_from_cache": ("rho", "fp32", ("k_rho_aoc2<1>", "k_rho<float, true, 1, 2>")),
 }
 
 

@@ -39,11 +39,11 @@ def test_stage_totals_and_copy_calibration(tmp_path, monkeypatch):
     fetch = [("__amd_rocclr_copyBuffer", 64, "FETCH_SIZE", 524288.0)]
     write = [("__amd_rocclr_copyBuffer", 64, "WRITE_SIZE", 1048576.0)]
     for _ in range(2):
-        fetch += [("chx::k_rho_aoc2(chx::RhoArgs)", 512, "FETCH_SIZE", 3000.0),
-                  ("chx::k_rho_aoc2(chx::RhoArgs)", 256, "FETCH_SIZE", 1000.0),
+        fetch += [("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 512, "FETCH_SIZE", 3000.0),
+                  ("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 256, "FETCH_SIZE", 1000.0),
                   ("void chx::k_rho<double, false, 1, 0>(chx::RhoArgs)", 512, "FETCH_SIZE", 50.0)]
-        write += [("chx::k_rho_aoc2(chx::RhoArgs)", 512, "WRITE_SIZE", 30.0),
-                  ("chx::k_rho_aoc2(chx::RhoArgs)", 256, "WRITE_SIZE", 10.0),
+        write += [("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 512, "WRITE_SIZE", 30.0),
+                  ("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 256, "WRITE_SIZE", 10.0),
                   ("void chx::k_rho<double, false, 1, 0>(chx::RhoArgs)", 512, "WRITE_SIZE", 5.0)]
         # V stage: two group kernels launched on the same grid size, one launch each per build
         fetch += [("void chx::k_vxc_group_pipe<17, true>(chx::VArgs)", 1024, "FETCH_SIZE", 700.0),
@@ -68,9 +68,9 @@ def test_stage_totals_and_copy_calibration(tmp_path, monkeypatch):
 
 
 def test_uneven_class_dispatches_are_refused(tmp_path):
-    rows = [("chx::k_rho_aoc2(chx::RhoArgs)", 512, "FETCH_SIZE", 1.0),
-            ("chx::k_rho_aoc2(chx::RhoArgs)", 256, "FETCH_SIZE", 1.0),
-            ("chx::k_rho_aoc2(chx::RhoArgs)", 512, "FETCH_SIZE", 1.0)]
+    rows = [("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 512, "FETCH_SIZE", 1.0),
+            ("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 256, "FETCH_SIZE", 1.0),
+            ("void chx::k_rho_aoc2<1>(chx::RhoArgs)", 512, "FETCH_SIZE", 1.0)]
     _write(str(tmp_path / "x"), rows)
     with pytest.raises(ValueError):
-        bench._per_stage(bench._pmc_rows(str(tmp_path / "x")), "FETCH_SIZE", ("k_rho_aoc2",))
+        bench._per_stage(bench._pmc_rows(str(tmp_path / "x")), "FETCH_SIZE", ("k_rho_aoc2<1>",))
