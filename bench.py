@@ -146,10 +146,16 @@ def sources_digest() -> str:
     return h.hexdigest()
 
 
-def load_traffic(workload: str, precision: str, kernel: str):
+def traffic_path(kernel: str, workload: str, precision: str, variant: str = "") -> str:
+    return os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{precision}{'_' + variant if variant else ''}_traffic.json")
+
+
+def load_traffic(workload: str, precision: str, kernel: str, variant: str = ""):
     """HBM bytes per build of the dominant kernel from the committed rocprofv3 PMC summary
-    (profiles/<kernel>_<workload>_<precision>_traffic.json), if it was measured on these sources."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_{precision}_traffic.json")
+    (profiles/<kernel>_<workload>_<precision>[_<variant>]_traffic.json), if it was measured on these
+    sources; variant names a plan option other than the default (vsmall_bf16: the BF16 small-group
+    V operands)."""
+    path = traffic_path(kernel, workload, precision, variant)
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -226,7 +232,7 @@ def _per_stage(rows, counter: str, kernels):
     return tot, n, stages
 
 
-def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str) -> None:
+def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str, variant: str = "") -> None:
     """profiles/<kernel>_<workload>_<prec>_traffic.json (kernel k_rho for the rho stage, k_vxc_group for
     the V stage) from one FETCH_SIZE and one WRITE_SIZE pass, each scaled by its reading for the 1 GiB
     device-to-device copy of the same run."""
@@ -255,7 +261,8 @@ def digest_traffic(fetch_dir: str, write_dir: str, workload: str, cmd: str) -> N
                "source": f"rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (one pass each) over {cmd}; sum over the "
                          f"{what} launches of one build (run total / stages), each counter scaled "
                          "by (1 GiB / its reading for the 1 GiB device-to-device copy kernel)"}
-        path = os.path.join(ROOT, "profiles", f"{kname}_{workload}_{prec}_traffic.json")
+        out["variant"] = variant
+        path = traffic_path(kname, workload, prec, variant)
         with open(path, "w") as f:
             json.dump(out, f, indent=1)
         log(f"wrote {path}: {out['hbm_bytes_per_build'] / 1e6:.1f} MB per build")
@@ -515,10 +522,12 @@ def main() -> int:
     ap.add_argument("--digest-valu", metavar="SQ_DIR", default=None,
                     help="write profiles/valu_<workload>.json from a rocprofv3 --pmc run of SQ counters, and exit")
     ap.add_argument("--digest-cmd", default="bench.py", help="the profiled command, recorded in the digest")
+    ap.add_argument("--digest-variant", default="", choices=["", "vsmall_bf16"],
+                    help="the plan option the counter runs used (vsmall_bf16: --v-small-bf16 2)")
     args = ap.parse_args()
     if args.digest_traffic or args.digest_valu:
         if args.digest_traffic:
-            digest_traffic(*args.digest_traffic, args.workload, args.digest_cmd)
+            digest_traffic(*args.digest_traffic, args.workload, args.digest_cmd, args.digest_variant)
         if args.digest_valu:
             digest_valu(args.digest_valu, args.workload, args.digest_cmd)
         return 0
@@ -639,9 +648,10 @@ def main() -> int:
     log(f"XC nuclear gradient: {grad_ms:.2f} ms, |sum over atoms| {np.abs(G.sum(axis=0)).max():.1e}")
 
     # Unrestricted build of the same workload (100 alpha / 99 beta orbitals), device buffers.
-    uks = UksVxcEngine(basis, grid, args.block_points, args.group_points,
-                       rho_tile_points=args.rho_tile_points, group_max_rows=args.group_max_rows,
-                       **plan_extra(args))
+    # (the mixed build's plan: its shapes and small-group V operand form)
+    ubp, ugp, ugr, uvs = shapes["fp32"]
+    uks = UksVxcEngine(basis, grid, ubp, ugp, rho_tile_points=args.rho_tile_points, group_max_rows=ugr,
+                       v_small_bf16=uvs, **plan_extra(args))
     Pa = 0.5 * P
     Pb = 0.5 * configs.random_density(basis, w["n_occ"] - 1, w["seed"] + 1, S)
     dPa, dPb = hr.DeviceArray.from_numpy(Pa), hr.DeviceArray.from_numpy(Pb)
@@ -704,10 +714,14 @@ def main() -> int:
     peak = PEAK_TFLOPS[best]
     measured_peak = measured_mfma_peak(best)
     log(f"measured {best} MFMA ceiling: {measured_peak:.1f} TFLOP/s (datasheet {peak})")
-    # the committed traffic digests are measured on the library's default plan; the rho stage's
+    # the committed traffic digests are measured on the library's default shapes; the rho stage's
     # traffic depends only on the blocks (block_points), the V stage's on the whole plan
-    dflt = shapes[best][0] == 0 if kname == "k_rho" else shapes[best] == (0, 0, 0, 0)
-    traffic = load_traffic(args.workload, best, kname) if dflt else None
+    if kname == "k_rho":
+        traffic = load_traffic(args.workload, best, kname) if shapes[best][0] == 0 else None
+    elif shapes[best][:3] == (0, 0, 0):
+        traffic = load_traffic(args.workload, best, kname, "vsmall_bf16" if shapes[best][3] == 2 else "")
+    else:
+        traffic = None
     build_tflops = (st["flops_rho"] + st["flops_v"]) / (rb["ms"] * 1e-3) / 1e12
     rho_hbm = (rho_cache_hbm_roofline(args.workload, st0, grid.n_points, n, kms["rho"], shapes["fp32"][0] == 0)
                if best == "fp32" else None)
