@@ -86,8 +86,17 @@ def autotune(basis: BasisSet, grid: MolecularGrid, func: str, P: np.ndarray, pre
         if e_first is None:
             e_first = e
         table.append(((bp, gp, gr, vs), ms, e))
+    best, ms = pick(table, e_tol, keep_first_within)
+    return TuneResult(best[0], best[1], best[2], ms, table, best[3])
+
+
+def pick(table, e_tol: float, keep_first_within: float = 0.0):
+    """(candidate, ms) of the fastest entry of table [(candidate, ms, E_xc)] whose E_xc is within
+    e_tol of the first entry's; the first entry itself unless that one is faster by more than
+    keep_first_within of the first entry's time."""
+    e_first = table[0][2]
     ok = [t for t in table if abs(t[2] - e_first) <= e_tol]
     best, ms, _ = min(ok, key=lambda t: t[1])
     if ms >= table[0][1] * (1.0 - keep_first_within):
         best, ms, _ = table[0]
-    return TuneResult(best[0], best[1], best[2], ms, table, best[3])
+    return best, ms
