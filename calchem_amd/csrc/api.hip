@@ -102,9 +102,10 @@ struct Plan {
         V, Vb_host, rho, rho_s, psub64, psub32, psub64b, psub32b, class_blocks, rho_sb, wv_b, mols, ao_mol, xc_wg,
         shell_atom,
         atom_xyz, inv_r, asym, raw_s, host_s, gwx_a, gwx_b, gfxc, gpart, gsum,
-        aoc, aoc_off;  // the mixed build's AO cache (empty: AOs evaluated per build)
+        aoc, aoc_off,  // the mixed build's AO cache (empty: AOs evaluated per build)
+        scratch_b, red_part_b;  // the second spin's V-stage tiles and reduction rows (unrestricted builds)
     int n_atoms = 0;  // atoms set by calchem_hip_vxc_set_atoms (0 = not set)
-    bool uks_ready = false;  // rho_sb / wv_b / psub64b / psub32b allocated (first unrestricted build)
+    bool uks_ready = false;  // rho_sb / wv_b / psub64b / psub32b / scratch_b / red_part_b allocated (first unrestricted build)
     long long psub_elems = 0;
     hipStream_t stream = nullptr;
     std::mutex mu;
@@ -133,6 +134,7 @@ struct Plan {
                              &pairs,    &ao_gbegin, &ao_groups, &scratch, &wv,     &wv_caller,
                              &epart,    &npart,     &en,       &P64,   &Pb_host,   &V,        &rho, &rho_s, &psub64, &psub32,
                              &psub64b, &psub32b, &class_blocks, &rho_sb, &wv_b, &Vb_host, &mols, &ao_mol, &xc_wg, &shell_atom, &atom_xyz,
+                             &scratch_b, &red_part_b,
                              &inv_r, &asym, &raw_s, &host_s, &gwx_a, &gwx_b, &gfxc, &gpart, &gsum, &aoc, &aoc_off};
         size_t s = 0;
         for (const DevBuf* x : b) s += x->bytes;
@@ -932,12 +934,14 @@ int run_rho(Plan& P, int mode, double4* sorted, const double* dP, hipStream_t s,
 // V stage: register-resident kernel over its groups, tile-pair kernel over
 // the rest, then the deterministic reduction into the full symmetric V.
 // With fused set this is part of a fused build and records its profiling mark.
-int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s, bool fused) {
+int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStream_t s, bool fused,
+          double* scratch = nullptr, double* red_part = nullptr) {
     if (f32 && !P.v_fits_fp32) return kLimit;
+    if (!scratch) scratch = P.scratch.as<double>();
+    if (!red_part) red_part = P.red_part.as<double>();
     VArgs va{P.shells.as<DevShell>(), P.prims.as<double2>(), P.aos.as<DevAo>(), P.pts.as<double4>(),
              P.groups.as<VGroup>(), P.pairs.as<VPair>(), P.group_list.as<int>(), P.group_rows.as<int2>(),
-             P.group_sh.as<int2>(), P.group_meta.as<uint4>(), wv, P.perm.as<int>(), wv_perm,
-             P.scratch.as<double>()};
+             P.group_sh.as<int2>(), P.group_meta.as<uint4>(), wv, P.perm.as<int>(), wv_perm, scratch};
     if (launch_vxc_group(va, f32, P.group_max_tiles, P.n_group_big, P.max_big_nsh, P.max_big_nprim, s) != hipSuccess)
         return kLaunchFail;
     VArgs vs = va;
@@ -952,8 +956,7 @@ int run_v(Plan& P, bool f32, const double4* wv, int wv_perm, double* dV, hipStre
     if (launch_vxc(va, f32, P.n_pairs, s) != hipSuccess) return kLaunchFail;
     if (fused) mark(P, s);
     ReduceArgs rd{P.ao_gbegin.as<int>(), P.ao_groups.as<RedEntry>(), P.group_ao.as<int>(), P.ao_mol.as<int>(),
-                  P.mols.as<MolInfo>(), P.n_ao, P.scratch.as<double>(), dV, P.reduce_segments,
-                  std::max(P.max_mol_ao, 1), P.red_part.as<double>()};
+                  P.mols.as<MolInfo>(), P.n_ao, scratch, dV, P.reduce_segments, std::max(P.max_mol_ao, 1), red_part};
     return launch_reduce_v(rd, P.reduce_waves, P.max_mol_ao, s) == hipSuccess ? kOk : kLaunchFail;
 }
 
@@ -996,7 +999,8 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
         const size_t b = sizeof(double4) * (size_t)std::max(P.n_pts, 1);
         if (!P.rho_sb.alloc(b) || !P.wv_b.alloc(b) ||
             (!P.psub64b.p && !P.psub64b.alloc(sizeof(double) * (size_t)(P.psub_elems + kPsubTail))) ||
-            !P.psub32b.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail)))
+            !P.psub32b.alloc(sizeof(float) * (size_t)(P.psub_elems + kPsubTail)) ||
+            !P.scratch_b.alloc(P.scratch.bytes) || !P.red_part_b.alloc(P.red_part.bytes))
             return kAllocFail;
         // Points of blocks without significant AOs are never written.
         if (hipMemset(P.rho_sb.p, 0, b) != hipSuccess) return kLaunchFail;
@@ -1012,8 +1016,13 @@ int run_build_uks(Plan& P, int func, int prec, const double* dPa, const double* 
                  P.wv.as<double4>(), P.wv_b.as<double4>(), P.epart.as<double>(), P.npart.as<double>()};
     if (launch_xc_uks(xa, s) != hipSuccess) return kLaunchFail;
     mark(P, s);
-    rc = run_v(P, f32, P.wv.as<double4>(), 0, dVa, s, false);
-    if (rc == kOk) rc = run_v(P, f32, P.wv_b.as<double4>(), 0, dVb, s, false);
+    // the two spins' V stages side by side (each with its own tiles and
+    // reduction rows), so that one's tail overlaps the other's launches
+    rc = fan_out(P, s, 2, [&](int i, hipStream_t st) {
+        return i == 0 ? run_v(P, f32, P.wv.as<double4>(), 0, dVa, st, false)
+                      : run_v(P, f32, P.wv_b.as<double4>(), 0, dVb, st, false, P.scratch_b.as<double>(),
+                              P.red_part_b.as<double>());
+    });
     if (rc != kOk) return rc;
     mark(P, s);
     mark(P, s);
